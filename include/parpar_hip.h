@@ -1,0 +1,180 @@
+/*
+ * parpar_hip — MI355X (gfx950) GF(2^16) recovery engine for ParPar.
+ *
+ * C ABI of libparpar_hip.so.  Plain pointers and sizes only; every entry point
+ * returns an int status (PPH_OK = 0, negative = error; pph_strerror() names it)
+ * unless stated otherwise, and no C++ exception crosses it.
+ *
+ * Two layers:
+ *
+ *   pph_proc_*  one recovery backend, entry for entry the plugin interface a
+ *               ParPar backend implements (IPAR2ProcBackend, gf16/controller.h:77-217),
+ *               with PAR2ProcOCL's semantics (gf16/controller_ocl.cpp) for staging,
+ *               batching and checksums.  Host buffers are caller-owned; an add
+ *               returns once the caller's buffer may be reused (the point where
+ *               PAR2ProcOCL's add future resolves, controller_ocl.cpp:296).
+ *               Calls on one pph_proc are not thread-safe (ParPar drives a
+ *               backend from one thread, controller.h:13-35).
+ *
+ *   pph_*       device-resident building blocks on HBM buffers (slot arrays),
+ *               used by the batched and serving drivers and the benchmark:
+ *               synthetic fill, input preparation (zero-pad + gf16_cksum +
+ *               log-domain conversion), the plain and dense mul-add kernels and
+ *               output checksum verification.
+ *
+ * Device slice format ("slot", one per input or recovery slice):
+ *   [slice data, slice_size bytes][zero][gf16_cksum block, cksum_size bytes at
+ *   ck_off = round_up(slice_size, 16)][zero to slot_bytes]
+ * The reference's OpenCL backend keeps the same [data][cksum] shape
+ * (gf16/controller_ocl.cpp:90,285-288); the checksum sits at a 16-byte aligned
+ * offset here so that an odd sub-slice never shares a GF(2^16) word with it.
+ * Plain-form inputs are stored in log domain after preparation: word x is
+ * replaced by log2(x) (x != 0) or 0xFFFF (x == 0).
+ */
+#ifndef PARPAR_HIP_H
+#define PARPAR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PPH_OK 0
+#define PPH_E_INVALID (-1)      /* argument out of range */
+#define PPH_E_NOMEM (-2)        /* device or pinned host allocation failed */
+#define PPH_E_STATE (-3)        /* call out of sequence */
+#define PPH_E_DEVICE (-4)       /* HIP runtime error; pph_last_hip_error() has the code */
+#define PPH_E_UNSUPPORTED (-5)  /* no HIP device / unsupported configuration */
+
+/* IPAR2ProcBackend::canAdd results (gf16/controller.h:38-43) */
+#define PPH_ADD_OK 0
+#define PPH_ADD_OK_BUSY 1
+#define PPH_ADD_FULL 2
+
+/* mul-add forms */
+#define PPH_FORM_AUTO 0
+#define PPH_FORM_PLAIN 1   /* LDS log/exp streaming kernel */
+#define PPH_FORM_DENSE 2   /* GF(2) bit-matrix contraction on int8 MFMA */
+
+const char* pph_strerror(int status);
+int pph_last_hip_error(void);
+int pph_device_count(void);
+/* Device name and CU count of a device; name may be NULL. */
+int pph_device_info(int device, char* name, size_t name_len, int* cu_count);
+
+/* ---- PAR2 coefficient helpers (gf16/gfmat_coeff.c:56-77) ---- */
+uint16_t pph_input_log(uint16_t input_num);                     /* gfmat_input_log */
+uint16_t pph_coeff_log(uint16_t input_log, uint16_t exponent);  /* gfmat_coeff_log */
+uint16_t pph_coeff(uint16_t input_num, uint16_t exponent);      /* gfmat_coeff */
+uint16_t pph_gf_mul(uint16_t a, uint16_t b);                    /* multiply in GF(2^16), 0x1100B */
+
+/* Slot geometry for a slice size and checksum width (8, 16, 32 or 64). */
+int pph_slot_layout(size_t slice_size, unsigned cksum_size, size_t* ck_off, size_t* slot_bytes);
+
+/* ======================================================================
+ * Backend layer: IPAR2ProcBackend (gf16/controller.h:77-217)
+ * ====================================================================== */
+typedef struct pph_proc pph_proc;
+
+/* Constructor: PAR2ProcOCL(platformId, deviceId, stagingAreas)
+ * (gf16/controller_ocl.cpp:27-63).  Fails with PPH_E_UNSUPPORTED without a device. */
+int pph_proc_create(int device, unsigned staging_areas, pph_proc** out);
+/* PAR2ProcOCL::init (gf16/controller_ocl.cpp:67-90): cksum_size is the checksum
+ * width of the CPU method the host selects (Galois16Mul::info().cksumSize,
+ * gf16/gf16mul.cpp:215-409; 64 on AVX512+GFNI hosts); input_batch 0 picks the
+ * default; form is a PPH_FORM_*.  setSliceSize must have been called. */
+int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_batch, int form);
+/* deinit / destructor (controller.h:198-203, controller_ocl.cpp:150-153) */
+void pph_proc_destroy(pph_proc* p);
+int pph_proc_set_slice_size(pph_proc* p, size_t slice_size);          /* setSliceSize (controller.h:160) */
+int pph_proc_set_current_slice_size(pph_proc* p, size_t slice_size);  /* setCurrentSliceSize (:166) */
+int pph_proc_set_recovery_slices(pph_proc* p, unsigned n, const uint16_t* exponents); /* (:167) */
+int pph_proc_can_add(const pph_proc* p);                              /* canAdd (:168) -> PPH_ADD_* */
+int pph_proc_wait_for_add(pph_proc* p);                               /* waitForAdd (:194) */
+/* addInput(buffer, size, inputNum, flush) (:169) */
+int pph_proc_add_input(pph_proc* p, const void* buffer, size_t size, uint16_t input_num, int flush);
+/* addInput(buffer, size, coeffs, flush) (:170): coeffs[r] for every recovery slice r */
+int pph_proc_add_input_coeffs(pph_proc* p, const void* buffer, size_t size, const uint16_t* coeffs, int flush);
+int pph_proc_dummy_input(pph_proc* p, uint16_t input_num, int flush); /* dummyInput (:171) */
+/* fillInput (:172): returns 1 once every staging area is filled, 0 otherwise */
+int pph_proc_fill_input(pph_proc* p, const void* buffer);
+int pph_proc_flush(pph_proc* p);                                      /* flush (:173) */
+/* endInput (:179): flushes and returns when every submitted batch has completed */
+int pph_proc_end_input(pph_proc* p);
+/* getOutput (:185): copies current_slice_size bytes of recovery slice `index`
+ * into output; *valid = gf16_cksum check of the slice (copy_cksum_check,
+ * gf16/gf16_cksum_base.h:148-159). */
+int pph_proc_get_output(pph_proc* p, unsigned index, void* output, int* valid);
+void pph_proc_discard_output(pph_proc* p);                            /* discardOutput (:186) */
+int pph_proc_has_added(const pph_proc* p);                            /* _hasAdded (:189) */
+int pph_proc_is_empty(const pph_proc* p);                             /* isEmpty (:182) */
+int pph_proc_free_processing_mem(pph_proc* p);                        /* freeProcessingMem (:203) */
+unsigned pph_proc_input_batch_size(const pph_proc* p);                /* getInputBatchSize (:205) */
+unsigned pph_proc_batches_started(const pph_proc* p);                 /* getBatchesStarted (:208) */
+void pph_proc_set_min_input_batch_size(pph_proc* p, unsigned n);      /* setMinInputBatchSize (:212) */
+int pph_proc_num_recovery_slices(const pph_proc* p);                  /* getNumRecoverySlices (:157) */
+/* Fault injection for tests: XOR `mask` into byte `offset` of recovery slice
+ * `index` on the device (the checksum must then report the slice invalid). */
+int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t offset, uint8_t mask);
+
+/* ======================================================================
+ * Device layer
+ * ====================================================================== */
+int pph_set_device(int device);
+int pph_malloc(void** ptr, size_t bytes);
+int pph_free(void* ptr);
+int pph_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int pph_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int pph_memset(void* dst, int value, size_t bytes);
+int pph_synchronize(void);
+/* Streams are opaque HIP streams at default priority; NULL = the null stream. */
+int pph_stream_create(void** stream);
+int pph_stream_destroy(void* stream);
+int pph_stream_synchronize(void* stream);
+/* Event timer on a stream: start/stop record events, elapsed waits for stop. */
+int pph_timer_create(void** timer);
+int pph_timer_destroy(void* timer);
+int pph_timer_start(void* timer, void* stream);
+int pph_timer_stop(void* timer, void* stream);
+int pph_timer_elapsed_ms(void* timer, float* ms);
+
+/* Synthetic input: slot k (k < count) of `slots` gets bytes [0, len) of the
+ * counter-based stream (seed, job, first_slice + k) and zeros to slot_bytes. */
+int pph_fill_random(void* slots, size_t slot_bytes, unsigned count, size_t len,
+                    uint64_t seed, uint32_t job, uint32_t first_slice, void* stream);
+
+/* Input preparation, the device half of copy_cksum (gf16/gf16_cksum_base.h:132-146):
+ * for slot k, bytes at or past lens[k] (device uint32 array; NULL = slice_size)
+ * read as zero, the gf16_cksum of the slice_size-byte slice is stored at ck_off,
+ * and, with to_log, every word of the slot is converted to log domain. */
+int pph_prep_slots(void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                   size_t slice_size, unsigned cksum_size, int to_log, void* stream);
+/* The same, reading data-form slots from `src` (a separate, non-overlapping
+ * slot array; NULL = in place) and writing the prepared slots to `dst`. */
+int pph_prep_slots_from(void* dst, const void* src, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                        size_t slice_size, unsigned cksum_size, int to_log, void* stream);
+
+/* Plain mul-add over log-domain input slots:
+ *   out[r] (^)= sum_i c(r,i) * in[i]   for r < n_out, i < n_in, over slot_bytes
+ * Coefficients are either c(r,i) = 2^(input_logs[i] * exps[r] mod 65535)
+ * (coef_log2 == NULL; input_logs/exps are device uint16 arrays), or explicit:
+ * coef_log2[i * coef_stride + r] = 2*log2(c), 0xFFFFFFFF for c == 0.
+ * accumulate = 0 overwrites the outputs (first batch), 1 XORs into them.
+ * group = outputs per pass (0 = auto). */
+int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in,
+                     const uint16_t* input_logs, const uint16_t* exps,
+                     const uint32_t* coef_log2, unsigned coef_stride,
+                     void* out_slots, unsigned n_out, int accumulate, unsigned group, void* stream);
+
+/* gf16_cksum verification of n slots (copy_cksum_check, gf16/gf16_cksum_base.h:148-159):
+ * valid[k] (device int array) = 1 if the stored block matches the slice data. */
+int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
+                     unsigned cksum_size, int* valid, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

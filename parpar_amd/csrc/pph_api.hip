@@ -1,0 +1,809 @@
+// C ABI of libparpar_hip.so: device-layer entry points and the pph_proc
+// recovery backend (the IPAR2ProcBackend contract, gf16/controller.h:77-217).
+#include "pph_internal.h"
+#include "../../include/parpar_hip.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace pph {
+
+static thread_local int g_last_hip = 0;
+
+int set_error(hipError_t e) {
+	g_last_hip = (int)e;
+	return PPH_E_DEVICE;
+}
+
+#define PPH_HIP(expr)                                  \
+	do {                                               \
+		hipError_t _e = (expr);                        \
+		if(_e != hipSuccess) return pph::set_error(_e); \
+	} while(0)
+
+// ---------------------------------------------------------------- host tables
+struct HostTables {
+	uint16_t exp[65536];
+	uint16_t log[65536];
+	uint16_t in_log[32768];
+	HostTables() {
+		uint32_t n = 1;
+		for(uint32_t e = 0; e < kOrder; e++) {
+			exp[e] = (uint16_t)n;
+			log[n] = (uint16_t)e;
+			n <<= 1;
+			if(n > 0xFFFF) n ^= kPoly;
+		}
+		exp[65535] = 0;        // zero entry of the lookup table
+		log[0] = kLogZero;     // log-domain encoding of 0
+		// PAR2 input constants: exponents n >= 1 not divisible by 3, 5, 17 or 257,
+		// in increasing order (gf16/gfmat_coeff.c:12-22)
+		uint32_t e = 0;
+		for(int i = 0; i < 32768; i++) {
+			do { e++; } while(!(e % 3) || !(e % 5) || !(e % 17) || !(e % 257));
+			in_log[i] = (uint16_t)e;
+		}
+	}
+};
+static const HostTables& tables() {
+	static HostTables t;
+	return t;
+}
+const uint16_t* host_exp() { return tables().exp; }
+const uint16_t* host_log() { return tables().log; }
+uint16_t host_input_log(uint16_t input) { return tables().in_log[input & 32767]; }
+uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent) {
+	uint32_t r = (uint32_t)input_log * exponent;
+	r = (r >> 16) + (r & 0xFFFF);
+	r += r >> 16;
+	r &= 0xFFFF;
+	return (uint16_t)(r == kOrder ? 0 : r);
+}
+
+// ---------------------------------------------------------------- device tables
+static std::mutex g_dev_mu;
+static DeviceTables g_dev[64];
+
+int device_tables(DeviceTables** out) {
+	int dev = 0;
+	PPH_HIP(hipGetDevice(&dev));
+	if(dev < 0 || dev >= 64) return PPH_E_UNSUPPORTED;
+	std::lock_guard<std::mutex> lk(g_dev_mu);
+	DeviceTables& t = g_dev[dev];
+	if(!t.exp) {
+		const HostTables& h = tables();
+		uint16_t *e = nullptr, *l = nullptr;
+		PPH_HIP(hipMalloc(&e, sizeof(h.exp)));
+		if(hipMalloc(&l, sizeof(h.log)) != hipSuccess) {
+			(void)hipFree(e);
+			return PPH_E_NOMEM;
+		}
+		PPH_HIP(hipMemcpy(e, h.exp, sizeof(h.exp), hipMemcpyHostToDevice));
+		PPH_HIP(hipMemcpy(l, h.log, sizeof(h.log), hipMemcpyHostToDevice));
+		hipDeviceProp_t prop;
+		PPH_HIP(hipGetDeviceProperties(&prop, dev));
+		t.cus = prop.multiProcessorCount;
+		t.log = l;
+		t.exp = e;
+	}
+	*out = &t;
+	return PPH_OK;
+}
+
+}  // namespace pph
+
+using namespace pph;
+
+// ====================================================================== misc ABI
+
+extern "C" const char* pph_strerror(int status) {
+	switch(status) {
+		case PPH_OK: return "ok";
+		case PPH_E_INVALID: return "invalid argument";
+		case PPH_E_NOMEM: return "out of memory";
+		case PPH_E_STATE: return "call out of sequence";
+		case PPH_E_DEVICE: return "HIP runtime error";
+		case PPH_E_UNSUPPORTED: return "unsupported (no HIP device or configuration)";
+		default: return "unknown status";
+	}
+}
+extern "C" int pph_last_hip_error(void) { return g_last_hip; }
+
+extern "C" int pph_device_count(void) {
+	int n = 0;
+	if(hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+extern "C" int pph_device_info(int device, char* name, size_t name_len, int* cu_count) {
+	hipDeviceProp_t prop;
+	PPH_HIP(hipGetDeviceProperties(&prop, device));
+	if(name && name_len) {
+		snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+	}
+	if(cu_count) *cu_count = prop.multiProcessorCount;
+	return PPH_OK;
+}
+
+extern "C" uint16_t pph_input_log(uint16_t input_num) { return host_input_log(input_num); }
+extern "C" uint16_t pph_coeff_log(uint16_t input_log, uint16_t exponent) {
+	return host_coeff_log(input_log, exponent);
+}
+extern "C" uint16_t pph_coeff(uint16_t input_num, uint16_t exponent) {
+	return host_exp()[host_coeff_log(host_input_log(input_num), exponent)];
+}
+extern "C" uint16_t pph_gf_mul(uint16_t a, uint16_t b) {
+	if(!a || !b) return 0;
+	uint32_t s = (uint32_t)host_log()[a] + host_log()[b];
+	if(s >= kOrder) s -= kOrder;
+	return host_exp()[s];
+}
+
+extern "C" int pph_slot_layout(size_t slice_size, unsigned cksum_size, size_t* ck_off, size_t* slot_bytes) {
+	if(!valid_ck(cksum_size)) return PPH_E_INVALID;
+	if(ck_off) *ck_off = ck_offset(slice_size);
+	if(slot_bytes) *slot_bytes = slot_size(slice_size, cksum_size);
+	return PPH_OK;
+}
+
+// ====================================================================== device layer
+
+extern "C" int pph_set_device(int device) {
+	PPH_HIP(hipSetDevice(device));
+	return PPH_OK;
+}
+extern "C" int pph_malloc(void** ptr, size_t bytes) {
+	if(!ptr) return PPH_E_INVALID;
+	hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+	if(e == hipErrorOutOfMemory) return PPH_E_NOMEM;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+extern "C" int pph_free(void* ptr) {
+	if(ptr) PPH_HIP(hipFree(ptr));
+	return PPH_OK;
+}
+extern "C" int pph_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+	if(bytes) PPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+	return PPH_OK;
+}
+extern "C" int pph_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+	if(bytes) PPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+	return PPH_OK;
+}
+extern "C" int pph_memset(void* dst, int value, size_t bytes) {
+	if(bytes) PPH_HIP(hipMemset(dst, value, bytes));
+	return PPH_OK;
+}
+extern "C" int pph_synchronize(void) {
+	PPH_HIP(hipDeviceSynchronize());
+	return PPH_OK;
+}
+extern "C" int pph_stream_create(void** stream) {
+	if(!stream) return PPH_E_INVALID;
+	hipStream_t s;
+	PPH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+	*stream = (void*)s;
+	return PPH_OK;
+}
+extern "C" int pph_stream_destroy(void* stream) {
+	if(stream) PPH_HIP(hipStreamDestroy((hipStream_t)stream));
+	return PPH_OK;
+}
+extern "C" int pph_stream_synchronize(void* stream) {
+	PPH_HIP(hipStreamSynchronize((hipStream_t)stream));
+	return PPH_OK;
+}
+
+struct pph_timer_t {
+	hipEvent_t a, b;
+};
+extern "C" int pph_timer_create(void** timer) {
+	if(!timer) return PPH_E_INVALID;
+	auto* t = new(std::nothrow) pph_timer_t;
+	if(!t) return PPH_E_NOMEM;
+	if(hipEventCreate(&t->a) != hipSuccess || hipEventCreate(&t->b) != hipSuccess) {
+		delete t;
+		return set_error(hipErrorUnknown);
+	}
+	*timer = t;
+	return PPH_OK;
+}
+extern "C" int pph_timer_destroy(void* timer) {
+	auto* t = (pph_timer_t*)timer;
+	if(!t) return PPH_OK;
+	(void)hipEventDestroy(t->a);
+	(void)hipEventDestroy(t->b);
+	delete t;
+	return PPH_OK;
+}
+extern "C" int pph_timer_start(void* timer, void* stream) {
+	PPH_HIP(hipEventRecord(((pph_timer_t*)timer)->a, (hipStream_t)stream));
+	return PPH_OK;
+}
+extern "C" int pph_timer_stop(void* timer, void* stream) {
+	PPH_HIP(hipEventRecord(((pph_timer_t*)timer)->b, (hipStream_t)stream));
+	return PPH_OK;
+}
+extern "C" int pph_timer_elapsed_ms(void* timer, float* ms) {
+	auto* t = (pph_timer_t*)timer;
+	PPH_HIP(hipEventSynchronize(t->b));
+	PPH_HIP(hipEventElapsedTime(ms, t->a, t->b));
+	return PPH_OK;
+}
+
+extern "C" int pph_fill_random(void* slots, size_t slot_bytes, unsigned count, size_t len, uint64_t seed,
+                               uint32_t job, uint32_t first_slice, void* stream) {
+	if(!slots || slot_bytes % 16 || len > slot_bytes) return PPH_E_INVALID;
+	PPH_HIP(launch_fill_random((uint8_t*)slots, slot_bytes, count, len, seed, job, first_slice, (hipStream_t)stream));
+	return PPH_OK;
+}
+
+extern "C" int pph_prep_slots_from(void* dst, const void* src, size_t slot_bytes, unsigned count,
+                                   const uint32_t* lens, size_t slice_size, unsigned cksum_size, int to_log,
+                                   void* stream) {
+	if(!dst || !valid_ck(cksum_size) || slot_bytes < slot_size(slice_size, cksum_size) || slot_bytes % kSlotAlign)
+		return PPH_E_INVALID;
+	if(src && src != dst) {
+		// distinct source and destination slot arrays must not overlap
+		const uint8_t *a = (const uint8_t*)src, *b = (const uint8_t*)dst;
+		const size_t n = slot_bytes * count;
+		if(a < b + n && b < a + n) return PPH_E_INVALID;
+	}
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	PPH_HIP(launch_prep((uint8_t*)dst, (const uint8_t*)src, slot_bytes, count, lens, nullptr, slice_size,
+	                    ck_offset(slice_size), cksum_size, to_log != 0, *t, (hipStream_t)stream));
+	return PPH_OK;
+}
+
+extern "C" int pph_prep_slots(void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                              size_t slice_size, unsigned cksum_size, int to_log, void* stream) {
+	return pph_prep_slots_from(slots, nullptr, slot_bytes, count, lens, slice_size, cksum_size, to_log, stream);
+}
+
+extern "C" int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                                const uint16_t* exps, const uint32_t* coef_log2, unsigned coef_stride,
+                                void* out_slots, unsigned n_out, int accumulate, unsigned group, void* stream) {
+	if((!in_slots && n_in) || !out_slots || slot_bytes % kSlotAlign) return PPH_E_INVALID;
+	if(group && group != 1 && group != 2 && group != 4 && group != 8 && group != 16) return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	hipError_t e = launch_muladd_plain((const uint16_t*)in_slots, slot_bytes / 2, n_in, input_logs, exps, coef_log2,
+	                                   coef_stride, (uint16_t*)out_slots, n_out, accumulate != 0, group, *t,
+	                                   (hipStream_t)stream);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+extern "C" int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
+                                unsigned cksum_size, int* valid, void* stream) {
+	if(!slots || !valid || !valid_ck(cksum_size) || slot_bytes < slot_size(slice_size, cksum_size) ||
+	   slot_bytes % kSlotAlign)
+		return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	PPH_HIP(launch_verify((const uint8_t*)slots, slot_bytes, count, slice_size, ck_offset(slice_size), cksum_size,
+	                      valid, *t, (hipStream_t)stream));
+	return PPH_OK;
+}
+
+// ====================================================================== backend
+
+// One staging area: `batch` device input slots, their pinned host bounce
+// buffers and per-slot coefficient data.  Active from submit until the
+// mul-add that consumes it has completed (the role of IPAR2ProcStaging's
+// isActive, gf16/controller.h:45-75).
+struct Area {
+	uint8_t* d_in = nullptr;        // batch * slot_bytes
+	uint8_t* h_stage = nullptr;     // pinned, batch * slice bytes
+	uint32_t* h_meta = nullptr;     // pinned: lens[batch] | slot idx[batch] | in_log[batch] (u16 packed)
+	uint32_t* d_meta = nullptr;
+	uint32_t* h_coef = nullptr;     // pinned explicit table batch * r_pad (or null)
+	uint32_t* d_coef = nullptr;
+	hipEvent_t done = nullptr;
+	bool active = false;
+	bool explicit_coef = false;
+	unsigned prep_count = 0;        // slots added since last submit needing preparation
+};
+
+struct pph_proc {
+	int device = 0;
+	hipStream_t s_copy = nullptr, s_comp = nullptr;
+	hipEvent_t ev_copied = nullptr;
+	DeviceTables* tab = nullptr;
+	unsigned n_areas = 2, batch = 0, min_batch = 0;
+	unsigned ck = 64;
+	int form = PPH_FORM_AUTO;
+	bool inited = false;
+	size_t slice_size = 0;       // allocated (setSliceSize)
+	size_t cur_size = 0;         // current (setCurrentSliceSize)
+	size_t slot_bytes = 0, alloc_slot_bytes = 0, alloc_slice = 0;
+	std::vector<Area> areas;
+	std::vector<uint16_t> exps;
+	unsigned r_pad = 0, r_alloc = 0;
+	uint8_t* d_out = nullptr;
+	uint16_t* d_exps = nullptr;
+	int* d_valid = nullptr;
+	std::vector<int> h_valid;
+	bool verified = false;
+	unsigned cur_area = 0, cur_inputs = 0;
+	unsigned active_count = 0;
+	unsigned batches_started = 0;
+	bool processing_add = false;
+};
+
+namespace {
+
+constexpr unsigned kDefaultBatch = 64;
+
+void meta_ptrs(pph_proc* p, Area& a, uint32_t** lens, uint32_t** slots, uint16_t** logs) {
+	*lens = a.h_meta;
+	*slots = a.h_meta + p->batch;
+	*logs = reinterpret_cast<uint16_t*>(a.h_meta + 2 * p->batch);
+}
+
+void free_areas(pph_proc* p) {
+	for(auto& a : p->areas) {
+		if(a.done) (void)hipEventSynchronize(a.done);
+		if(a.d_in) (void)hipFree(a.d_in);
+		if(a.h_stage) (void)hipHostFree(a.h_stage);
+		if(a.h_meta) (void)hipHostFree(a.h_meta);
+		if(a.d_meta) (void)hipFree(a.d_meta);
+		if(a.h_coef) (void)hipHostFree(a.h_coef);
+		if(a.d_coef) (void)hipFree(a.d_coef);
+		if(a.done) (void)hipEventDestroy(a.done);
+		a = Area();
+	}
+}
+
+void free_outputs(pph_proc* p) {
+	if(p->d_out) (void)hipFree(p->d_out);
+	if(p->d_exps) (void)hipFree(p->d_exps);
+	if(p->d_valid) (void)hipFree(p->d_valid);
+	p->d_out = nullptr;
+	p->d_exps = nullptr;
+	p->d_valid = nullptr;
+	p->r_alloc = 0;
+}
+
+// refresh isActive flags from the device
+void poll_areas(const pph_proc* cp) {
+	pph_proc* p = const_cast<pph_proc*>(cp);
+	for(auto& a : p->areas) {
+		if(a.active && hipEventQuery(a.done) == hipSuccess) {
+			a.active = false;
+			p->active_count--;
+		}
+	}
+}
+
+int wait_area(pph_proc* p, Area& a) {
+	if(a.active) {
+		PPH_HIP(hipEventSynchronize(a.done));
+		a.active = false;
+		p->active_count--;
+	}
+	return PPH_OK;
+}
+
+int alloc_areas(pph_proc* p) {
+	free_areas(p);
+	p->areas.resize(p->n_areas);
+	const size_t stage = p->slice_size ? p->slice_size : 1;
+	for(auto& a : p->areas) {
+		if(hipMalloc(&a.d_in, (size_t)p->batch * p->slot_bytes) != hipSuccess) return PPH_E_NOMEM;
+		if(hipHostMalloc(&a.h_stage, (size_t)p->batch * stage, hipHostMallocDefault) != hipSuccess) return PPH_E_NOMEM;
+		const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
+		if(hipHostMalloc(&a.h_meta, meta, hipHostMallocDefault) != hipSuccess) return PPH_E_NOMEM;
+		if(hipMalloc(&a.d_meta, meta) != hipSuccess) return PPH_E_NOMEM;
+		PPH_HIP(hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
+	}
+	p->alloc_slot_bytes = p->slot_bytes;
+	p->alloc_slice = p->slice_size;
+	return PPH_OK;
+}
+
+int ensure_coef_tables(pph_proc* p) {
+	for(auto& a : p->areas) {
+		if(a.h_coef) (void)hipHostFree(a.h_coef);
+		if(a.d_coef) (void)hipFree(a.d_coef);
+		a.h_coef = nullptr;
+		a.d_coef = nullptr;
+		if(!p->r_pad) continue;
+		const size_t bytes = (size_t)p->batch * p->r_pad * sizeof(uint32_t);
+		if(hipHostMalloc(&a.h_coef, bytes, hipHostMallocDefault) != hipSuccess) return PPH_E_NOMEM;
+		if(hipMalloc(&a.d_coef, bytes) != hipSuccess) return PPH_E_NOMEM;
+	}
+	return PPH_OK;
+}
+
+// submit the current area's `n` inputs (PAR2ProcOCL::run_kernel, controller_ocl.cpp:496-527)
+int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
+	Area& a = p->areas[area_idx];
+	uint32_t *lens, *slots;
+	uint16_t* logs;
+	meta_ptrs(p, a, &lens, &slots, &logs);
+	const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
+	PPH_HIP(hipMemcpyAsync(a.d_meta, a.h_meta, meta, hipMemcpyHostToDevice, p->s_copy));
+	if(a.prep_count) {
+		PPH_HIP(launch_prep(a.d_in, nullptr, p->slot_bytes, a.prep_count, a.d_meta, a.d_meta + p->batch, p->cur_size,
+		                    ck_offset(p->cur_size), p->ck, true, *p->tab, p->s_copy));
+		a.prep_count = 0;
+	}
+	if(a.explicit_coef) {
+		PPH_HIP(hipMemcpyAsync(a.d_coef, a.h_coef, (size_t)p->batch * p->r_pad * sizeof(uint32_t),
+		                       hipMemcpyHostToDevice, p->s_copy));
+	}
+	PPH_HIP(hipEventRecord(p->ev_copied, p->s_copy));
+	PPH_HIP(hipStreamWaitEvent(p->s_comp, p->ev_copied, 0));
+	if(!p->exps.empty()) {
+		const uint16_t* d_logs = reinterpret_cast<const uint16_t*>(a.d_meta + 2 * p->batch);
+		hipError_t e = launch_muladd_plain(reinterpret_cast<const uint16_t*>(a.d_in), p->slot_bytes / 2, n, d_logs,
+		                                   p->d_exps, a.explicit_coef ? a.d_coef : nullptr, p->r_pad, (uint16_t*)p->d_out,
+		                                   (unsigned)p->exps.size(), p->processing_add, 0, *p->tab, p->s_comp);
+		if(e != hipSuccess) return set_error(e);
+		p->processing_add = true;
+	}
+	PPH_HIP(hipEventRecord(a.done, p->s_comp));
+	a.active = true;
+	a.explicit_coef = false;
+	p->active_count++;
+	p->batches_started++;
+	p->verified = false;
+	return PPH_OK;
+}
+
+void advance_area(pph_proc* p) {
+	p->cur_inputs = 0;
+	if(++p->cur_area == p->areas.size()) p->cur_area = 0;
+}
+
+bool should_submit(const pph_proc* p, bool flush) {
+	return flush || p->cur_inputs == p->batch ||
+	       (p->active_count == 0 && p->areas.size() > 1 && p->cur_inputs >= p->min_batch);
+}
+
+// write one input's coefficient data into the area (set_coeffs, controller_ocl.cpp:387-403)
+void set_coeffs_num(pph_proc* p, Area& a, unsigned idx, uint16_t input_num) {
+	uint32_t *lens, *slots;
+	uint16_t* logs;
+	meta_ptrs(p, a, &lens, &slots, &logs);
+	const uint16_t lg = host_input_log(input_num);
+	logs[idx] = lg;
+	if(a.explicit_coef) {
+		for(unsigned r = 0; r < p->r_pad; r++)
+			a.h_coef[(size_t)idx * p->r_pad + r] =
+			    r < p->exps.size() ? 2u * host_coeff_log(lg, p->exps[r]) : kSkipCoef;
+	}
+}
+
+// switch an area to an explicit table, materialising the coefficients of the
+// inputs already staged in it
+void to_explicit(pph_proc* p, Area& a, unsigned upto) {
+	if(a.explicit_coef) return;
+	a.explicit_coef = true;
+	uint32_t *lens, *slots;
+	uint16_t* logs;
+	meta_ptrs(p, a, &lens, &slots, &logs);
+	for(unsigned k = 0; k < upto; k++)
+		for(unsigned r = 0; r < p->r_pad; r++)
+			a.h_coef[(size_t)k * p->r_pad + r] =
+			    r < p->exps.size() ? 2u * host_coeff_log(logs[k], p->exps[r]) : kSkipCoef;
+}
+
+int add_common(pph_proc* p, const void* buffer, size_t size, const uint16_t* coeffs, uint16_t input_num, int flush,
+               bool dummy) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(!dummy && size && !buffer) return PPH_E_INVALID;
+	if(size > p->cur_size) return PPH_E_INVALID;
+	Area& a = p->areas[p->cur_area];
+	int rc = wait_area(p, a);
+	if(rc) return rc;
+	const unsigned idx = p->cur_inputs;
+	uint32_t *lens, *slots;
+	uint16_t* logs;
+	meta_ptrs(p, a, &lens, &slots, &logs);
+	if(coeffs) {
+		if(!a.h_coef) return PPH_E_STATE;
+		to_explicit(p, a, idx);
+		for(unsigned r = 0; r < p->r_pad; r++) {
+			uint32_t v = kSkipCoef;
+			if(r < p->exps.size() && coeffs[r]) v = 2u * host_log()[coeffs[r]];
+			a.h_coef[(size_t)idx * p->r_pad + r] = v;
+		}
+		logs[idx] = 0;
+	} else {
+		set_coeffs_num(p, a, idx, input_num);
+	}
+	if(!dummy) {
+		uint8_t* stage = a.h_stage + (size_t)idx * p->alloc_slice;
+		if(size) memcpy(stage, buffer, size);
+		if(size)
+			PPH_HIP(hipMemcpyAsync(a.d_in + (size_t)idx * p->slot_bytes, stage, size, hipMemcpyHostToDevice,
+			                       p->s_copy));
+		lens[a.prep_count] = (uint32_t)size;
+		slots[a.prep_count] = idx;
+		a.prep_count++;
+	}
+	p->cur_inputs++;
+	poll_areas(p);
+	if(should_submit(p, flush != 0)) {
+		const unsigned area = p->cur_area;
+		rc = run_kernel(p, area, p->cur_inputs);
+		advance_area(p);
+		if(rc) return rc;
+	}
+	// the caller's buffer has been copied: it may be reused now
+	return PPH_OK;
+}
+
+}  // namespace
+
+extern "C" int pph_proc_create(int device, unsigned staging_areas, pph_proc** out) {
+	if(!out) return PPH_E_INVALID;
+	*out = nullptr;
+	int n = 0;
+	if(hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PPH_E_UNSUPPORTED;
+	if(device < 0 || device >= n) return PPH_E_INVALID;
+	auto* p = new(std::nothrow) pph_proc;
+	if(!p) return PPH_E_NOMEM;
+	p->device = device;
+	p->n_areas = staging_areas ? staging_areas : 2;
+	hipError_t e = hipSetDevice(device);
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&p->s_copy, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&p->s_comp, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_copied, hipEventDisableTiming);
+	if(e != hipSuccess) {
+		pph_proc_destroy(p);
+		return set_error(e);
+	}
+	int rc = device_tables(&p->tab);
+	if(rc) {
+		pph_proc_destroy(p);
+		return rc;
+	}
+	*out = p;
+	return PPH_OK;
+}
+
+extern "C" void pph_proc_destroy(pph_proc* p) {
+	if(!p) return;
+	(void)hipSetDevice(p->device);
+	if(p->s_comp) (void)hipStreamSynchronize(p->s_comp);
+	if(p->s_copy) (void)hipStreamSynchronize(p->s_copy);
+	free_areas(p);
+	free_outputs(p);
+	if(p->ev_copied) (void)hipEventDestroy(p->ev_copied);
+	if(p->s_copy) (void)hipStreamDestroy(p->s_copy);
+	if(p->s_comp) (void)hipStreamDestroy(p->s_comp);
+	delete p;
+}
+
+extern "C" int pph_proc_set_slice_size(pph_proc* p, size_t slice_size) {
+	if(!p) return PPH_E_INVALID;
+	if(slice_size > 0xFFFFFFFFull) return PPH_E_INVALID;
+	p->slice_size = slice_size;
+	if(!p->inited) p->cur_size = slice_size;
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_batch, int form) {
+	if(!p || !valid_ck(cksum_size)) return PPH_E_INVALID;
+	if(form != PPH_FORM_AUTO && form != PPH_FORM_PLAIN) return PPH_E_UNSUPPORTED;
+	PPH_HIP(hipSetDevice(p->device));
+	if(p->s_comp) PPH_HIP(hipStreamSynchronize(p->s_comp));
+	p->ck = cksum_size;
+	p->form = form;
+	p->batch = input_batch ? input_batch : kDefaultBatch;
+	p->min_batch = p->batch;
+	p->cur_size = p->slice_size;
+	p->slot_bytes = slot_size(p->cur_size, p->ck);
+	p->exps.clear();
+	free_outputs(p);
+	int rc = alloc_areas(p);
+	if(rc) return rc;
+	p->cur_area = p->cur_inputs = 0;
+	p->active_count = 0;
+	p->batches_started = 0;
+	p->processing_add = false;
+	p->verified = false;
+	p->r_pad = 0;
+	p->inited = true;
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_set_current_slice_size(pph_proc* p, size_t slice_size) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(slice_size > 0xFFFFFFFFull) return PPH_E_INVALID;
+	// only between passes: every staged batch must have finished
+	if(p->cur_inputs) return PPH_E_STATE;
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	poll_areas(p);
+	const size_t slot = slot_size(slice_size, p->ck);
+	p->cur_size = slice_size;
+	p->slot_bytes = slot;
+	if(slot > p->alloc_slot_bytes || slice_size > p->alloc_slice) {
+		// grow (controller_ocl.cpp:227-240 re-inits in that case)
+		p->slice_size = std::max(p->slice_size, slice_size);
+		int rc = alloc_areas(p);
+		if(rc) return rc;
+		rc = ensure_coef_tables(p);
+		if(rc) return rc;
+		if(p->r_alloc) {
+			auto ex = p->exps;
+			free_outputs(p);
+			p->exps.clear();
+			return pph_proc_set_recovery_slices(p, (unsigned)ex.size(), ex.data());
+		}
+	}
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_set_recovery_slices(pph_proc* p, unsigned n, const uint16_t* exponents) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(n > 65535) return PPH_E_INVALID;
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	poll_areas(p);
+	p->exps.assign(n, 1);  // default exponent 1 (controller_cpu.cpp:176)
+	if(exponents) {
+		for(unsigned r = 0; r < n; r++) {
+			if(exponents[r] == 65535) return PPH_E_INVALID;  // gfmat_coeff_log's precondition
+			p->exps[r] = exponents[r];
+		}
+	}
+	if(!n) return PPH_OK;
+	const unsigned group = 16;
+	const unsigned r_pad = (n + group - 1) / group * group;
+	if(!p->d_out || r_pad > p->r_alloc || p->slot_bytes > p->alloc_slot_bytes) {
+		free_outputs(p);
+		if(hipMalloc(&p->d_out, (size_t)r_pad * p->alloc_slot_bytes) != hipSuccess) return PPH_E_NOMEM;
+		if(hipMalloc(&p->d_exps, (size_t)r_pad * sizeof(uint16_t)) != hipSuccess) return PPH_E_NOMEM;
+		if(hipMalloc(&p->d_valid, (size_t)r_pad * sizeof(int)) != hipSuccess) return PPH_E_NOMEM;
+		p->r_alloc = r_pad;
+	}
+	if(r_pad != p->r_pad) {
+		p->r_pad = r_pad;
+		int rc = ensure_coef_tables(p);
+		if(rc) return rc;
+	}
+	std::vector<uint16_t> ex(r_pad, 0);
+	std::copy(p->exps.begin(), p->exps.end(), ex.begin());
+	PPH_HIP(hipMemcpy(p->d_exps, ex.data(), r_pad * sizeof(uint16_t), hipMemcpyHostToDevice));
+	p->h_valid.assign(r_pad, 0);
+	p->verified = false;
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_can_add(const pph_proc* p) {
+	if(!p || !p->inited) return PPH_ADD_FULL;
+	poll_areas(p);
+	if(p->areas[p->cur_area].active) return PPH_ADD_FULL;
+	return p->active_count < p->areas.size() - 1 ? PPH_ADD_OK : PPH_ADD_OK_BUSY;
+}
+
+extern "C" int pph_proc_wait_for_add(pph_proc* p) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	return wait_area(p, p->areas[p->cur_area]);
+}
+
+extern "C" int pph_proc_add_input(pph_proc* p, const void* buffer, size_t size, uint16_t input_num, int flush) {
+	return add_common(p, buffer, size, nullptr, input_num, flush, false);
+}
+extern "C" int pph_proc_add_input_coeffs(pph_proc* p, const void* buffer, size_t size, const uint16_t* coeffs,
+                                         int flush) {
+	if(!coeffs) return PPH_E_INVALID;
+	return add_common(p, buffer, size, coeffs, 0, flush, false);
+}
+extern "C" int pph_proc_dummy_input(pph_proc* p, uint16_t input_num, int flush) {
+	return add_common(p, nullptr, 0, nullptr, input_num, flush, true);
+}
+
+extern "C" int pph_proc_fill_input(pph_proc* p, const void* buffer) {
+	// PAR2ProcOCL::fillInput (controller_ocl.cpp:366-380): prepare the next slot of
+	// the current area in place, without scheduling any processing
+	if(!p || !p->inited || !buffer) return PPH_E_INVALID;
+	Area& a = p->areas[p->cur_area];
+	int rc = wait_area(p, a);
+	if(rc) return rc;
+	uint8_t* stage = a.h_stage + (size_t)p->cur_inputs * p->alloc_slice;
+	memcpy(stage, buffer, p->cur_size);
+	uint8_t* dst = a.d_in + (size_t)p->cur_inputs * p->slot_bytes;
+	PPH_HIP(hipMemcpyAsync(dst, stage, p->cur_size, hipMemcpyHostToDevice, p->s_copy));
+	PPH_HIP(launch_prep(dst, nullptr, p->slot_bytes, 1, nullptr, nullptr, p->cur_size, ck_offset(p->cur_size), p->ck, true,
+	                    *p->tab, p->s_copy));
+	PPH_HIP(hipStreamSynchronize(p->s_copy));
+	if(++p->cur_inputs == p->batch) {
+		p->cur_inputs = 0;
+		if(++p->cur_area == p->areas.size()) {
+			p->cur_area = 0;
+			return 1;
+		}
+	}
+	return 0;
+}
+
+extern "C" int pph_proc_flush(pph_proc* p) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(!p->cur_inputs) return PPH_OK;
+	const unsigned area = p->cur_area;
+	int rc = run_kernel(p, area, p->cur_inputs);
+	advance_area(p);
+	return rc;
+}
+
+extern "C" int pph_proc_end_input(pph_proc* p) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	int rc = pph_proc_flush(p);
+	if(rc) return rc;
+	PPH_HIP(hipStreamSynchronize(p->s_copy));
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	poll_areas(p);
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_get_output(pph_proc* p, unsigned index, void* output, int* valid) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(index >= p->exps.size() || (!output && p->cur_size)) return PPH_E_INVALID;
+	if(!p->processing_add) {
+		// nothing computed on this backend: zero fill (controller.cpp:376-382)
+		if(p->cur_size) memset(output, 0, p->cur_size);
+		if(valid) *valid = 1;
+		return PPH_OK;
+	}
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	if(!p->verified) {
+		PPH_HIP(launch_verify(p->d_out, p->slot_bytes, (unsigned)p->exps.size(), p->cur_size, ck_offset(p->cur_size),
+		                      p->ck, p->d_valid, *p->tab, p->s_comp));
+		PPH_HIP(hipMemcpyAsync(p->h_valid.data(), p->d_valid, p->exps.size() * sizeof(int), hipMemcpyDeviceToHost,
+		                       p->s_comp));
+		PPH_HIP(hipStreamSynchronize(p->s_comp));
+		p->verified = true;
+	}
+	if(p->cur_size)
+		PPH_HIP(hipMemcpy(output, p->d_out + (size_t)index * p->slot_bytes, p->cur_size, hipMemcpyDeviceToHost));
+	if(valid) *valid = p->h_valid[index];
+	return PPH_OK;
+}
+
+extern "C" void pph_proc_discard_output(pph_proc* p) {
+	if(p) p->processing_add = false;
+}
+extern "C" int pph_proc_has_added(const pph_proc* p) { return p && p->processing_add ? 1 : 0; }
+extern "C" int pph_proc_is_empty(const pph_proc* p) {
+	if(!p) return 1;
+	poll_areas(p);
+	return p->active_count == 0 ? 1 : 0;
+}
+extern "C" int pph_proc_free_processing_mem(pph_proc* p) {
+	if(!p) return PPH_E_INVALID;
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	poll_areas(p);
+	free_outputs(p);
+	p->exps.clear();
+	p->processing_add = false;
+	return PPH_OK;
+}
+extern "C" unsigned pph_proc_input_batch_size(const pph_proc* p) { return p ? p->batch : 0; }
+extern "C" unsigned pph_proc_batches_started(const pph_proc* p) { return p ? p->batches_started : 0; }
+extern "C" void pph_proc_set_min_input_batch_size(pph_proc* p, unsigned n) {
+	if(p) p->min_batch = n ? std::min(n, p->batch) : p->batch;
+}
+extern "C" int pph_proc_num_recovery_slices(const pph_proc* p) { return p ? (int)p->exps.size() : 0; }
+
+extern "C" int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t offset, uint8_t mask) {
+	if(!p || !p->inited || index >= p->exps.size() || offset >= p->slot_bytes) return PPH_E_INVALID;
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	PPH_HIP(launch_xor_byte(p->d_out + (size_t)index * p->slot_bytes + offset, mask, p->s_comp));
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
+	p->verified = false;
+	return PPH_OK;
+}

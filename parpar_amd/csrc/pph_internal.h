@@ -1,0 +1,65 @@
+// Internal declarations shared by the kernel and API translation units of
+// libparpar_hip.so (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace pph {
+
+// GF(2^16) with the PAR2 polynomial x^16 + x^12 + x^3 + x + 1 (0x1100B),
+// generator 2 (gf16/gf16_global.h:9-10, gf16/gfmat_coeff.c:12-19).
+constexpr uint32_t kPoly = 0x1100B;
+constexpr uint32_t kOrder = 65535;
+// Byte address of the zero entry of the LDS exp table and the "no contribution"
+// marker of explicit coefficient tables.
+constexpr uint32_t kZeroIdx2 = 2u * 65535u;
+constexpr uint32_t kSkipCoef = 0xFFFFFFFFu;
+// Log-domain encoding of the zero word.
+constexpr uint16_t kLogZero = 0xFFFF;
+// Decoded log of zero: large enough that both candidates of the modular
+// reduction exceed kZeroIdx2 for every coefficient.
+constexpr uint32_t kLogZeroDecoded = 0x40000000u;
+
+constexpr unsigned kSlotAlign = 1024;   // slot size granularity (bytes)
+constexpr unsigned kPlainThreads = 1024;
+constexpr unsigned kPlainWordsPerLane = 8;
+
+struct DeviceTables {
+	uint16_t* exp = nullptr;   // exp[k] = 2^k for k < 65535; exp[65535] = 0
+	uint16_t* log = nullptr;   // log[x] = log2(x) for x != 0; log[0] = 0xFFFF
+	int cus = 0;
+};
+
+// Host-side tables (built once).
+const uint16_t* host_exp();
+const uint16_t* host_log();
+uint16_t host_input_log(uint16_t input);
+uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent);
+
+// Per-device tables, uploaded on first use of a device (current device).
+int device_tables(DeviceTables** out);
+
+int set_error(hipError_t e);
+
+// Kernel launchers (pph_kernels.hip); all return hipError_t.
+hipError_t launch_fill_random(uint8_t* slots, size_t slot_bytes, unsigned count, size_t len,
+                              uint64_t seed, uint32_t job, uint32_t first_slice, hipStream_t s);
+hipError_t launch_prep(uint8_t* slots, const uint8_t* src, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                       const uint32_t* slot_index, size_t slice_size, size_t ck_off, unsigned ck,
+                       bool to_log, const DeviceTables& t, hipStream_t s);
+hipError_t launch_verify(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
+                         size_t ck_off, unsigned ck, int* valid, const DeviceTables& t, hipStream_t s);
+hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n_in,
+                               const uint16_t* in_logs, const uint16_t* exps,
+                               const uint32_t* coef, unsigned coef_stride,
+                               uint16_t* out, unsigned n_out, bool accumulate, unsigned group,
+                               const DeviceTables& t, hipStream_t s);
+hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
+
+inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline size_t ck_offset(size_t slice_size) { return round_up(slice_size, 16); }
+inline size_t slot_size(size_t slice_size, unsigned ck) { return round_up(ck_offset(slice_size) + ck, kSlotAlign); }
+inline bool valid_ck(unsigned ck) { return ck == 8 || ck == 16 || ck == 32 || ck == 64; }
+
+}  // namespace pph

@@ -1,0 +1,444 @@
+// HIP kernels for gfx950: synthetic fill, input preparation (zero padding,
+// gf16_cksum, log-domain conversion), the plain mul-add and checksum
+// verification.  Launchers validate shapes on the host before launching.
+#include "pph_internal.h"
+
+namespace pph {
+
+// ------------------------------------------------------------------ helpers
+
+// Two GF(2^16) lanes packed in a dword, each multiplied by 2 (x) modulo 0x1100B:
+// the lane-wise Horner step of gf16_cksum (gf16/gf16_checksum_generic.h:25-39).
+__device__ __forceinline__ uint32_t mul2x2(uint32_t v) {
+	return ((v << 1) & 0xFFFEFFFEu) ^ (((v >> 15) & 0x00010001u) * (kPoly & 0xFFFFu));
+}
+
+// v * 2^e for one lane, e < 65535, through the global log/exp tables.
+__device__ __forceinline__ uint32_t mul_pow2(uint32_t v, uint32_t e, const uint16_t* __restrict__ lg,
+                                             const uint16_t* __restrict__ ex) {
+	if(!v) return 0;
+	uint32_t s = (uint32_t)lg[v] + e;
+	if(s >= kOrder) s -= kOrder;
+	return ex[s];
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+__host__ __device__ inline uint64_t slice_key(uint64_t seed, uint32_t job, uint32_t slice) {
+	auto m = [](uint64_t z) {
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+		return z ^ (z >> 31);
+	};
+	uint64_t k = m(seed * kGolden + 0x632BE59BD9B4E019ull);
+	return m(k ^ (((uint64_t)job << 32) | (uint64_t)slice));
+}
+
+// byte mask keeping positions < len of the dword starting at byte `pos`
+__device__ __forceinline__ uint32_t keep_mask(size_t pos, size_t len) {
+	if(pos + 4 <= len) return 0xFFFFFFFFu;
+	if(pos >= len) return 0u;
+	return (1u << (8 * (unsigned)(len - pos))) - 1u;
+}
+
+// ------------------------------------------------------------------ fill
+
+__global__ void k_fill_random(uint8_t* __restrict__ slots, size_t slot_bytes, unsigned count, size_t len,
+                              uint64_t seed, uint32_t job, uint32_t first_slice) {
+	const size_t pieces_per_slot = slot_bytes / 16;
+	const size_t total = pieces_per_slot * count;
+	for(size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+		const unsigned k = (unsigned)(p / pieces_per_slot);
+		const size_t off = (p - (size_t)k * pieces_per_slot) * 16;
+		uint4 v = make_uint4(0, 0, 0, 0);
+		if(off < len) {
+			const uint64_t key = slice_key(seed, job, first_slice + k);
+			const uint64_t w = off >> 3;
+			const uint64_t a = mix64(key + (w + 1) * kGolden);
+			const uint64_t b = mix64(key + (w + 2) * kGolden);
+			v.x = (uint32_t)a & keep_mask(off, len);
+			v.y = (uint32_t)(a >> 32) & keep_mask(off + 4, len);
+			v.z = (uint32_t)b & keep_mask(off + 8, len);
+			v.w = (uint32_t)(b >> 32) & keep_mask(off + 12, len);
+		}
+		*reinterpret_cast<uint4*>(slots + (size_t)k * slot_bytes + off) = v;
+	}
+}
+
+hipError_t launch_fill_random(uint8_t* slots, size_t slot_bytes, unsigned count, size_t len,
+                              uint64_t seed, uint32_t job, uint32_t first_slice, hipStream_t s) {
+	if(!count) return hipSuccess;
+	const size_t total = slot_bytes / 16 * count;
+	size_t blocks = (total + 255) / 256;
+	if(blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_fill_random, dim3((unsigned)blocks), dim3(256), 0, s, slots, slot_bytes, count, len, seed,
+	                   job, first_slice);
+	return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ checksum
+
+// gf16_cksum of a slot on the device.  The slice is a sequence of CK-byte rows
+// (zero padded); the checksum is the lane-wise Horner sum
+//     ck = sum_k 2^(nb-1-k) * row_k,   nb = ceil(slice_size / CK)
+// which is what gf16_cksum_copy stores (gf16/gf16_cksum_base.h:132-146,
+// including the 2^n step for zero rows after a short input).  One workgroup per
+// slot: each group of TPR threads runs Horner over a contiguous run of rows, the
+// partial sums are scaled by 2^(rows after the run) and XOR-reduced in LDS.
+constexpr unsigned kCkThreads = 256;
+
+template <unsigned CK>
+struct CkGeom {
+	static constexpr unsigned P = CK < 16 ? CK : 16;  // bytes per thread per row
+	static constexpr unsigned TPR = CK / P;           // threads per row
+	static constexpr unsigned NG = kCkThreads / TPR;  // row groups
+	static constexpr unsigned DW = P / 4;             // dwords per piece
+};
+
+template <unsigned DW>
+struct Piece {
+	uint32_t d[DW];
+};
+
+template <unsigned DW>
+__device__ __forceinline__ Piece<DW> load_piece(const uint8_t* p) {
+	Piece<DW> r;
+	if constexpr(DW == 4) {
+		uint4 v = *reinterpret_cast<const uint4*>(p);
+		r.d[0] = v.x; r.d[1] = v.y; r.d[2] = v.z; r.d[3] = v.w;
+	} else {
+		uint2 v = *reinterpret_cast<const uint2*>(p);
+		r.d[0] = v.x; r.d[1] = v.y;
+	}
+	return r;
+}
+template <unsigned DW>
+__device__ __forceinline__ void store_piece(uint8_t* p, const Piece<DW>& r) {
+	if constexpr(DW == 4) {
+		*reinterpret_cast<uint4*>(p) = make_uint4(r.d[0], r.d[1], r.d[2], r.d[3]);
+	} else {
+		*reinterpret_cast<uint2*>(p) = make_uint2(r.d[0], r.d[1]);
+	}
+}
+
+// Horner partials of one workgroup over slot rows [0, nrows) of `src`; rows >= nb
+// do not enter the checksum.  MODE 0: verify (read only), 1: prep to data form
+// (write the zero-padded slice to `slot`), 2: prep to log domain (write every
+// word of `slot` in log domain).  src may equal slot.
+template <unsigned CK, int MODE>
+__device__ void ck_rows(const uint8_t* src, uint8_t* slot, size_t nrows, size_t nb, size_t len, uint32_t* h,
+                        const uint16_t* __restrict__ lg, size_t& run_end) {
+	const bool inplace = src == slot;
+	using G = CkGeom<CK>;
+	const unsigned t = threadIdx.x % G::TPR;
+	const unsigned g = threadIdx.x / G::TPR;
+	const size_t rpg = (nrows + G::NG - 1) / G::NG;
+	const size_t r0 = (size_t)g * rpg;
+	size_t r1 = r0 + rpg;
+	if(r1 > nrows) r1 = nrows;
+	for(unsigned j = 0; j < G::DW; j++) h[j] = 0;
+	for(size_t row = r0; row < r1; row++) {
+		const size_t off = row * CK + (size_t)t * G::P;
+		uint8_t* p = slot + off;
+		Piece<G::DW> v = load_piece<G::DW>(src + off);
+		const bool partial = off + G::P > len;
+		if(partial) {
+			for(unsigned j = 0; j < G::DW; j++) v.d[j] &= keep_mask(off + 4 * j, len);
+		}
+		if(row < nb) {
+			for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]) ^ v.d[j];
+		}
+		if constexpr(MODE == 1) {
+			if(partial || !inplace) store_piece<G::DW>(p, v);
+		} else if constexpr(MODE == 2) {
+			Piece<G::DW> o;
+			for(unsigned j = 0; j < G::DW; j++) {
+				uint32_t lo = lg[v.d[j] & 0xFFFFu];
+				uint32_t hi = lg[v.d[j] >> 16];
+				o.d[j] = lo | (hi << 16);
+			}
+			store_piece<G::DW>(p, o);
+		}
+	}
+	run_end = r1 < nb ? r1 : nb;
+	if(r0 >= nb) run_end = (size_t)-1;  // run contributes nothing
+}
+
+// Scale this thread's partial to its final position and reduce across groups.
+// Returns (in threads < CK/2) the checksum lane `threadIdx.x`.
+template <unsigned CK>
+__device__ uint32_t ck_reduce(const uint32_t* h, size_t nb, size_t run_end, const uint16_t* __restrict__ lg,
+                              const uint16_t* __restrict__ ex) {
+	using G = CkGeom<CK>;
+	__shared__ uint16_t red[G::NG][CK / 2];
+	const unsigned t = threadIdx.x % G::TPR;
+	const unsigned g = threadIdx.x / G::TPR;
+	const uint32_t e = run_end == (size_t)-1 ? 0 : (uint32_t)((nb - run_end) % kOrder);
+	for(unsigned j = 0; j < G::DW; j++) {
+		uint32_t lo = 0, hi = 0;
+		if(run_end != (size_t)-1) {
+			lo = mul_pow2(h[j] & 0xFFFFu, e, lg, ex);
+			hi = mul_pow2(h[j] >> 16, e, lg, ex);
+		}
+		red[g][t * (G::P / 2) + 2 * j] = (uint16_t)lo;
+		red[g][t * (G::P / 2) + 2 * j + 1] = (uint16_t)hi;
+	}
+	__syncthreads();
+	uint32_t acc = 0;
+	if(threadIdx.x < CK / 2) {
+		for(unsigned k = 0; k < G::NG; k++) acc ^= red[k][threadIdx.x];
+	}
+	return acc;
+}
+
+template <unsigned CK>
+__global__ __launch_bounds__(kCkThreads) void k_prep(uint8_t* slots, const uint8_t* src_slots, size_t slot_bytes,
+                                                     const uint32_t* __restrict__ lens,
+                                                     const uint32_t* __restrict__ slot_index, size_t slice_size,
+                                                     size_t ck_off, int to_log, const uint16_t* __restrict__ lg,
+                                                     const uint16_t* __restrict__ ex) {
+	const unsigned k = blockIdx.x;
+	const size_t sidx = slot_index ? slot_index[k] : k;
+	uint8_t* slot = slots + sidx * slot_bytes;
+	const uint8_t* src = src_slots ? src_slots + sidx * slot_bytes : slot;
+	size_t len = lens ? lens[k] : slice_size;
+	if(len > slice_size) len = slice_size;
+	const size_t nrows = slot_bytes / CK;
+	const size_t nb = (slice_size + CK - 1) / CK;
+	uint32_t h[CkGeom<CK>::DW];
+	size_t run_end;
+	if(to_log)
+		ck_rows<CK, 2>(src, slot, nrows, nb, len, h, lg, run_end);
+	else
+		ck_rows<CK, 1>(src, slot, nrows, nb, len, h, lg, run_end);
+	__threadfence();  // row rewrites complete before the checksum block is stored over them
+	uint32_t lane = ck_reduce<CK>(h, nb, run_end, lg, ex);
+	if(threadIdx.x < CK / 2) {
+		uint16_t v = (uint16_t)lane;
+		if(to_log) v = lg[v];
+		*reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = v;
+	}
+}
+
+template <unsigned CK>
+__global__ __launch_bounds__(kCkThreads) void k_verify(const uint8_t* __restrict__ slots, size_t slot_bytes,
+                                                       size_t slice_size, size_t ck_off, int* __restrict__ valid,
+                                                       const uint16_t* __restrict__ lg,
+                                                       const uint16_t* __restrict__ ex) {
+	const unsigned k = blockIdx.x;
+	uint8_t* slot = const_cast<uint8_t*>(slots) + (size_t)k * slot_bytes;
+	const size_t nb = (slice_size + CK - 1) / CK;
+	uint32_t h[CkGeom<CK>::DW];
+	size_t run_end;
+	ck_rows<CK, 0>(slot, slot, nb, nb, slice_size, h, lg, run_end);
+	uint32_t lane = ck_reduce<CK>(h, nb, run_end, lg, ex);
+	__shared__ int bad;
+	if(threadIdx.x == 0) bad = 0;
+	__syncthreads();
+	if(threadIdx.x < CK / 2) {
+		uint16_t stored = *reinterpret_cast<const uint16_t*>(slot + ck_off + 2 * threadIdx.x);
+		if(stored != (uint16_t)lane) atomicOr(&bad, 1);
+	}
+	__syncthreads();
+	if(threadIdx.x == 0) valid[k] = bad ? 0 : 1;
+}
+
+hipError_t launch_prep(uint8_t* slots, const uint8_t* src, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                       const uint32_t* slot_index, size_t slice_size, size_t ck_off, unsigned ck, bool to_log,
+                       const DeviceTables& t, hipStream_t s) {
+	if(!count) return hipSuccess;
+	if(slot_bytes % kSlotAlign || ck_off + ck > slot_bytes || slice_size > ck_off || !valid_ck(ck))
+		return hipErrorInvalidValue;
+	dim3 grid(count), block(kCkThreads);
+	switch(ck) {
+		case 8: hipLaunchKernelGGL(k_prep<8>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
+		case 16: hipLaunchKernelGGL(k_prep<16>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
+		case 32: hipLaunchKernelGGL(k_prep<32>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
+		default: hipLaunchKernelGGL(k_prep<64>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_verify(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size, size_t ck_off,
+                         unsigned ck, int* valid, const DeviceTables& t, hipStream_t s) {
+	if(!count) return hipSuccess;
+	if(slot_bytes % kSlotAlign || ck_off + ck > slot_bytes || slice_size > ck_off || !valid_ck(ck))
+		return hipErrorInvalidValue;
+	dim3 grid(count), block(kCkThreads);
+	switch(ck) {
+		case 8: hipLaunchKernelGGL(k_verify<8>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
+		case 16: hipLaunchKernelGGL(k_verify<16>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
+		case 32: hipLaunchKernelGGL(k_verify<32>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
+		default: hipLaunchKernelGGL(k_verify<64>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
+	}
+	return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ plain mul-add
+
+// out[r] (^)= sum_i c(r,i) * in[i] in GF(2^16).  Inputs are in log domain.
+// Each lane owns 8 consecutive words of a 8192-word column tile and G recovery
+// slices; the full 64 Ki-entry exp table sits in LDS (entry 65535 = 0), so one
+// product is   exp[min(lx + lc, lx + lc - 65535, 65535)]   with doubled
+// (byte-address) logs: two adds and a min3, no branch for zero words.
+// Coefficients are uniform across the wavefront: either computed from the
+// input log and the recovery exponent on the scalar unit (PAR2 coefficients,
+// gf16/gfmat_coeff.c:60-77) or read from an explicit table.
+__device__ __forceinline__ uint32_t mod65535(uint32_t x) {
+	x = (x & 0xFFFFu) + (x >> 16);
+	x = (x & 0xFFFFu) + (x >> 16);
+	return x;  // <= 65535, congruent; 65535 acts as 0 in the lookup below
+}
+
+__device__ __forceinline__ uint32_t exp_pair(const uint16_t* s_exp, uint32_t a, uint32_t b) {
+	uint32_t lo = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + a);
+	uint32_t hi = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + b);
+	return lo | (hi << 16);
+}
+
+template <int G, bool TABLE>
+__global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
+    const uint16_t* __restrict__ in, size_t slot_words, unsigned n_in, const uint16_t* __restrict__ in_logs,
+    const uint16_t* __restrict__ exps, const uint32_t* __restrict__ coef, unsigned coef_stride,
+    uint16_t* __restrict__ out, unsigned n_out, int accumulate, const uint16_t* __restrict__ exp_tab,
+    unsigned col_tiles, unsigned groups) {
+	__shared__ uint16_t s_exp[65536];
+	{
+		const uint4* src = reinterpret_cast<const uint4*>(exp_tab);
+		uint4* dst = reinterpret_cast<uint4*>(s_exp);
+		for(unsigned k = threadIdx.x; k < 65536 * 2 / 16; k += kPlainThreads) dst[k] = src[k];
+	}
+	__syncthreads();
+
+	// blocks of one column tile run on the same XCD (blockIdx round-robins over 8 XCDs)
+	const unsigned total = col_tiles * groups;
+	unsigned L = blockIdx.x;
+	if((total & 7u) == 0) L = (blockIdx.x & 7u) * (total >> 3) + (blockIdx.x >> 3);
+	const unsigned col = L / groups;
+	const unsigned grp = L - col * groups;
+	const size_t w0 = (size_t)col * (kPlainThreads * kPlainWordsPerLane) + (size_t)threadIdx.x * kPlainWordsPerLane;
+	const bool active = w0 < slot_words;
+	const unsigned r0 = grp * G;
+
+	uint32_t e[G];
+	if constexpr(!TABLE) {
+#pragma unroll
+		for(int r = 0; r < G; r++) e[r] = (r0 + r < n_out) ? exps[r0 + r] : 0u;
+	}
+
+	uint32_t acc[G][4];
+#pragma unroll
+	for(int r = 0; r < G; r++) {
+		acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
+		if(accumulate && active && r0 + r < n_out) {
+			uint4 v = *reinterpret_cast<const uint4*>(out + (size_t)(r0 + r) * slot_words + w0);
+			acc[r][0] = v.x; acc[r][1] = v.y; acc[r][2] = v.z; acc[r][3] = v.w;
+		}
+	}
+
+	for(unsigned i = 0; i < n_in; i++) {
+		uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+		if(active) v = *reinterpret_cast<const uint4*>(in + (size_t)i * slot_words + w0);
+		const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+		uint32_t lx[8], lxm[8];
+#pragma unroll
+		for(int j = 0; j < 4; j++) {
+			const uint32_t L0 = wv[j] & 0xFFFFu, L1 = wv[j] >> 16;
+			lx[2 * j] = L0 == kLogZero ? kLogZeroDecoded : 2 * L0;
+			lx[2 * j + 1] = L1 == kLogZero ? kLogZeroDecoded : 2 * L1;
+			lxm[2 * j] = lx[2 * j] - kZeroIdx2;
+			lxm[2 * j + 1] = lx[2 * j + 1] - kZeroIdx2;
+		}
+		uint32_t lin = 0;
+		if constexpr(!TABLE) lin = in_logs[i];
+#pragma unroll
+		for(int r = 0; r < G; r++) {
+			uint32_t lc;
+			bool use;
+			if constexpr(TABLE) {
+				lc = coef[(size_t)i * coef_stride + r0 + r];
+				use = lc != kSkipCoef;
+			} else {
+				lc = 2 * mod65535(lin * e[r]);
+				use = r0 + r < n_out;
+			}
+			if(use) {
+#pragma unroll
+				for(int j = 0; j < 4; j++) {
+					const uint32_t a = min(min(lx[2 * j] + lc, lxm[2 * j] + lc), kZeroIdx2);
+					const uint32_t b = min(min(lx[2 * j + 1] + lc, lxm[2 * j + 1] + lc), kZeroIdx2);
+					acc[r][j] ^= exp_pair(s_exp, a, b);
+				}
+			}
+		}
+	}
+
+	if(active) {
+#pragma unroll
+		for(int r = 0; r < G; r++) {
+			if(r0 + r < n_out)
+				*reinterpret_cast<uint4*>(out + (size_t)(r0 + r) * slot_words + w0) =
+				    make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+		}
+	}
+}
+
+template <int G>
+static hipError_t launch_plain_g(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
+                                 const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, uint16_t* out,
+                                 unsigned n_out, bool accumulate, const DeviceTables& t, hipStream_t s) {
+	const unsigned tile_words = kPlainThreads * kPlainWordsPerLane;
+	const unsigned col_tiles = (unsigned)((slot_words + tile_words - 1) / tile_words);
+	const unsigned groups = (n_out + G - 1) / G;
+	dim3 grid(col_tiles * groups), block(kPlainThreads);
+	if(coef)
+		hipLaunchKernelGGL((k_muladd_plain<G, true>), grid, block, 0, s, in, slot_words, n_in, in_logs, exps, coef,
+		                   coef_stride, out, n_out, accumulate ? 1 : 0, t.exp, col_tiles, groups);
+	else
+		hipLaunchKernelGGL((k_muladd_plain<G, false>), grid, block, 0, s, in, slot_words, n_in, in_logs, exps, coef,
+		                   coef_stride, out, n_out, accumulate ? 1 : 0, t.exp, col_tiles, groups);
+	return hipGetLastError();
+}
+
+hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
+                               const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, uint16_t* out,
+                               unsigned n_out, bool accumulate, unsigned group, const DeviceTables& t,
+                               hipStream_t s) {
+	if(!n_out || !slot_words) return hipSuccess;
+	if(slot_words % (kSlotAlign / 2)) return hipErrorInvalidValue;
+	if(!coef && (!in_logs || !exps) && n_in) return hipErrorInvalidValue;
+	if(group == 0) {
+		group = 16;
+		while(group > 1 && group / 2 >= n_out) group /= 2;
+	}
+	// an explicit table must cover every output group it is read for
+	if(coef) {
+		const unsigned padded = (n_out + group - 1) / group * group;
+		if(coef_stride < padded) return hipErrorInvalidValue;
+	}
+	switch(group) {
+		case 1: return launch_plain_g<1>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		case 2: return launch_plain_g<2>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		case 4: return launch_plain_g<4>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		case 8: return launch_plain_g<8>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		case 16: return launch_plain_g<16>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		default: return hipErrorInvalidValue;
+	}
+}
+
+// ------------------------------------------------------------------ misc
+
+__global__ void k_xor_byte(uint8_t* p, uint8_t mask) {
+	if(threadIdx.x == 0 && blockIdx.x == 0) *p ^= mask;
+}
+hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s) {
+	hipLaunchKernelGGL(k_xor_byte, dim3(1), dim3(64), 0, s, p, mask);
+	return hipGetLastError();
+}
+
+}  // namespace pph

@@ -1,0 +1,91 @@
+"""The C ABI library: it loads, exports every symbol include/parpar_hip.h
+declares, and its host-side helpers agree with the golden vectors.  No compute
+call needs a GPU here."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import parpar_amd
+from parpar_amd import proc as P
+from golden_io import manifest
+
+G = manifest()
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = parpar_amd.lib()
+    names = parpar_amd.declared_symbols()
+    assert len(names) > 40
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", parpar_amd.LIB_PATH], stdout=subprocess.PIPE,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # the ctypes signature table covers the whole header
+    assert set(names) <= set(parpar_amd._SIGS)
+
+
+def test_library_is_built_for_gfx950():
+    data = open(parpar_amd.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_host_coefficient_helpers_match_reference_vectors():
+    L = parpar_amd.lib()
+    for k, v in G["input_log"].items():
+        assert L.pph_input_log(int(k)) == v
+    for k, v in G["coeff"].items():
+        i, e = map(int, k.split(","))
+        assert L.pph_coeff(i, e) == v
+    OL = O.lib()
+    rng = np.random.default_rng(2)
+    for a, b in rng.integers(0, 65536, (5000, 2)):
+        assert L.pph_gf_mul(int(a), int(b)) == OL.po_mul_slow(int(a), int(b))
+    for i in range(0, 32768, 37):
+        lg = L.pph_input_log(i)
+        for e in (0, 1, 2, 65534, 40000):
+            assert L.pph_coeff_log(lg, e) == OL.po_coeff_log(lg, e)
+
+
+def test_slot_layout():
+    for S in (0, 1, 2, 100, 1001, 65536, 1048576, 16 * 1048576 + 2):
+        for ck in (8, 16, 32, 64):
+            off, slot = P.slot_layout(S, ck)
+            assert off >= S and off % 16 == 0 and off - S < 16
+            assert slot % 1024 == 0 and slot >= off + ck and slot - (off + ck) < 1024
+    with pytest.raises(parpar_amd.PPHError):
+        P.slot_layout(100, 24)
+
+
+def test_status_strings():
+    L = parpar_amd.lib()
+    for s in (0, -1, -2, -3, -4, -5):
+        assert L.pph_strerror(s)
+
+
+def test_no_device_reports_unsupported():
+    if parpar_amd.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    p = ctypes.c_void_p()
+    assert parpar_amd.lib().pph_proc_create(0, 2, ctypes.byref(p)) == parpar_amd.PPH_E_UNSUPPORTED
+    with pytest.raises(RuntimeError):
+        parpar_amd.require_device()
+
+
+def test_controller_allocation_rules():
+    # PAR2Proc::checkBackendAllocation (gf16/controller.cpp:42-84): even offsets,
+    # full cover, odd size only for the last range
+    class Dummy:
+        def setSliceSize(self, s):
+            self.s = s
+
+    c = P.PAR2Proc()
+    assert c.init(100, [(Dummy(), 0, 50), (Dummy(), 50, 50)])
+    assert not c.init(100, [(Dummy(), 0, 51), (Dummy(), 51, 49)])
+    assert not c.init(100, [(Dummy(), 0, 50), (Dummy(), 52, 48)])
+    assert c.init(101, [(Dummy(), 0, 50), (Dummy(), 50, 51)])
