@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# DRAM bytes per k_muladd_plain launch measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs, scaled by a 1 GiB plain copy in the same runs
+# (profiles/k_muladd_plain_2000x1MiB_R200.md), keyed by (inputs, slice, recovery).
+MEASURED_TRAFFIC = {(2000, 1 << 20, 200): 19.21e9 + 0.210e9}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -46,6 +51,8 @@ def parse():
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run one plain device copy of this many bytes (HBM counter reference)")
     return ap.parse_args()
 
 
@@ -65,12 +72,12 @@ def cpu_baseline(args, gpu_out_window, window_bytes_used):
         out = O.recover(inp, [win] * n, win, exps, threads=threads)
         return time.perf_counter() - t0, out
 
-    # calibrate, then size the sample to about cpu_seconds of work
-    cal = 4096
-    t, _ = run(cal)
-    per_byte = t / cal
-    win = int(args.cpu_seconds / max(per_byte, 1e-12))
-    win = max(cal, min(win, window_bytes_used, S))
+    # calibrate on one 32 KiB column tile (one OpenMP work item), then size the
+    # window so that `threads` tiles in parallel take about cpu_seconds
+    tile = 32768
+    t, _ = run(tile)
+    tiles = int(args.cpu_seconds * threads / max(t, 1e-9))
+    win = max(tile, min(tiles * tile, window_bytes_used, S))
     win -= win % 1024
     log("cpu baseline: %d threads, window %d bytes of each slice" % (threads, win))
     t, out = run(win)
@@ -148,6 +155,15 @@ def main():
         chk(L.pph_stream_synchronize(stream))
         log("warmup %d done" % (w + 1))
 
+    if args.calib_copy:
+        cbytes = args.calib_copy - args.calib_copy % 16
+        ca = P.DeviceBuffer(cbytes)
+        cb = P.DeviceBuffer(cbytes)
+        chk(L.pph_copy(cb.ptr, ca.ptr, cbytes, stream), "copy")
+        chk(L.pph_stream_synchronize(stream))
+        log("calibration copy of %d bytes done" % cbytes)
+        del ca, cb
+
     ker_ms = []
     chk(L.pph_synchronize())
     t0 = time.perf_counter()
@@ -202,7 +218,8 @@ def main():
             "mul_adds": ma, "device": name.value.decode(), "cus": cus.value},
         "ma_per_s": round(ma / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((n, S, R)),
                      "kernel": "k_muladd_plain", "kernel_ms": round(ker_avg, 3),
                      "algorithmic_bytes": 2 * ma},
         "cpu_baseline": cpu,

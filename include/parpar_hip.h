@@ -129,6 +129,9 @@ int pph_free(void* ptr);
 int pph_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int pph_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int pph_memset(void* dst, int value, size_t bytes);
+/* Device-to-device copy by a plain 16-byte streaming kernel (bytes and both
+ * pointers 16-byte aligned); the reference point for HBM traffic counters. */
+int pph_copy(void* dst, const void* src, size_t bytes, void* stream);
 int pph_synchronize(void);
 /* Streams are opaque HIP streams at default priority; NULL = the null stream. */
 int pph_stream_create(void** stream);

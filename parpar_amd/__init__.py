@@ -78,6 +78,7 @@ _SIGS = {
     "pph_memcpy_h2d": (_i, [_vp, _vp, _sz]),
     "pph_memcpy_d2h": (_i, [_vp, _vp, _sz]),
     "pph_memset": (_i, [_vp, _i, _sz]),
+    "pph_copy": (_i, [_vp, _vp, _sz, _vp]),
     "pph_synchronize": (_i, []),
     "pph_stream_create": (_i, [ctypes.POINTER(_vp)]),
     "pph_stream_destroy": (_i, [_vp]),

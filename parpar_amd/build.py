@@ -65,6 +65,7 @@ def build_oracle(verbose=False):
     _run(["make", "-C", odir, "liboracle.so"])
     if os.path.isdir("/root/reference/gf16"):
         _run(["make", "-C", odir, "-j8", "ref"])
+        _run(["make", "-C", odir, "integration"])
     if verbose:
         print("built oracle")
 

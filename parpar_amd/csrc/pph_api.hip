@@ -175,6 +175,11 @@ extern "C" int pph_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 	if(bytes) PPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 	return PPH_OK;
 }
+extern "C" int pph_copy(void* dst, const void* src, size_t bytes, void* stream) {
+	if(!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return PPH_E_INVALID;
+	PPH_HIP(launch_copy(dst, src, bytes, (hipStream_t)stream));
+	return PPH_OK;
+}
 extern "C" int pph_memset(void* dst, int value, size_t bytes) {
 	if(bytes) PPH_HIP(hipMemset(dst, value, bytes));
 	return PPH_OK;

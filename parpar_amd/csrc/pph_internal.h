@@ -56,6 +56,7 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
                                uint16_t* out, unsigned n_out, bool accumulate, unsigned group,
                                const DeviceTables& t, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline size_t ck_offset(size_t slice_size) { return round_up(slice_size, 16); }

@@ -90,13 +90,14 @@ hipError_t launch_fill_random(uint8_t* slots, size_t slot_bytes, unsigned count,
 // including the 2^n step for zero rows after a short input).  One workgroup per
 // slot: each group of TPR threads runs Horner over a contiguous run of rows, the
 // partial sums are scaled by 2^(rows after the run) and XOR-reduced in LDS.
-constexpr unsigned kCkThreads = 256;
+constexpr unsigned kCkThreads = 256;     // verification: one workgroup per slot
+constexpr unsigned kPrepThreads = 1024;  // preparation: persistent, log table in LDS
 
-template <unsigned CK>
+template <unsigned CK, unsigned T>
 struct CkGeom {
 	static constexpr unsigned P = CK < 16 ? CK : 16;  // bytes per thread per row
 	static constexpr unsigned TPR = CK / P;           // threads per row
-	static constexpr unsigned NG = kCkThreads / TPR;  // row groups
+	static constexpr unsigned NG = T / TPR;           // row groups
 	static constexpr unsigned DW = P / 4;             // dwords per piece
 };
 
@@ -129,12 +130,14 @@ __device__ __forceinline__ void store_piece(uint8_t* p, const Piece<DW>& r) {
 // Horner partials of one workgroup over slot rows [0, nrows) of `src`; rows >= nb
 // do not enter the checksum.  MODE 0: verify (read only), 1: prep to data form
 // (write the zero-padded slice to `slot`), 2: prep to log domain (write every
-// word of `slot` in log domain).  src may equal slot.
-template <unsigned CK, int MODE>
+// word of `slot` in log domain through the table `lg`).  src may equal slot.
+// Rows are read four at a time so that several loads are in flight per thread.
+template <unsigned CK, unsigned T, int MODE>
 __device__ void ck_rows(const uint8_t* src, uint8_t* slot, size_t nrows, size_t nb, size_t len, uint32_t* h,
-                        const uint16_t* __restrict__ lg, size_t& run_end) {
+                        const uint16_t* lg, size_t& run_end) {
+	using G = CkGeom<CK, T>;
+	constexpr unsigned U = 4;
 	const bool inplace = src == slot;
-	using G = CkGeom<CK>;
 	const unsigned t = threadIdx.x % G::TPR;
 	const unsigned g = threadIdx.x / G::TPR;
 	const size_t rpg = (nrows + G::NG - 1) / G::NG;
@@ -142,27 +145,35 @@ __device__ void ck_rows(const uint8_t* src, uint8_t* slot, size_t nrows, size_t 
 	size_t r1 = r0 + rpg;
 	if(r1 > nrows) r1 = nrows;
 	for(unsigned j = 0; j < G::DW; j++) h[j] = 0;
-	for(size_t row = r0; row < r1; row++) {
-		const size_t off = row * CK + (size_t)t * G::P;
-		uint8_t* p = slot + off;
-		Piece<G::DW> v = load_piece<G::DW>(src + off);
-		const bool partial = off + G::P > len;
-		if(partial) {
-			for(unsigned j = 0; j < G::DW; j++) v.d[j] &= keep_mask(off + 4 * j, len);
+	for(size_t rb = r0; rb < r1; rb += U) {
+		Piece<G::DW> v[U];
+#pragma unroll
+		for(unsigned u = 0; u < U; u++) {
+			if(rb + u < r1) v[u] = load_piece<G::DW>(src + (rb + u) * CK + (size_t)t * G::P);
 		}
-		if(row < nb) {
-			for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]) ^ v.d[j];
-		}
-		if constexpr(MODE == 1) {
-			if(partial || !inplace) store_piece<G::DW>(p, v);
-		} else if constexpr(MODE == 2) {
-			Piece<G::DW> o;
-			for(unsigned j = 0; j < G::DW; j++) {
-				uint32_t lo = lg[v.d[j] & 0xFFFFu];
-				uint32_t hi = lg[v.d[j] >> 16];
-				o.d[j] = lo | (hi << 16);
+#pragma unroll
+		for(unsigned u = 0; u < U; u++) {
+			const size_t row = rb + u;
+			if(row >= r1) break;
+			const size_t off = row * CK + (size_t)t * G::P;
+			const bool partial = off + G::P > len;
+			if(partial) {
+				for(unsigned j = 0; j < G::DW; j++) v[u].d[j] &= keep_mask(off + 4 * j, len);
 			}
-			store_piece<G::DW>(p, o);
+			if(row < nb) {
+				for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]) ^ v[u].d[j];
+			}
+			if constexpr(MODE == 1) {
+				if(partial || !inplace) store_piece<G::DW>(slot + off, v[u]);
+			} else if constexpr(MODE == 2) {
+				Piece<G::DW> o;
+				for(unsigned j = 0; j < G::DW; j++) {
+					uint32_t lo = lg[v[u].d[j] & 0xFFFFu];
+					uint32_t hi = lg[v[u].d[j] >> 16];
+					o.d[j] = lo | (hi << 16);
+				}
+				store_piece<G::DW>(slot + off, o);
+			}
 		}
 	}
 	run_end = r1 < nb ? r1 : nb;
@@ -171,10 +182,10 @@ __device__ void ck_rows(const uint8_t* src, uint8_t* slot, size_t nrows, size_t 
 
 // Scale this thread's partial to its final position and reduce across groups.
 // Returns (in threads < CK/2) the checksum lane `threadIdx.x`.
-template <unsigned CK>
+template <unsigned CK, unsigned T>
 __device__ uint32_t ck_reduce(const uint32_t* h, size_t nb, size_t run_end, const uint16_t* __restrict__ lg,
                               const uint16_t* __restrict__ ex) {
-	using G = CkGeom<CK>;
+	using G = CkGeom<CK, T>;
 	__shared__ uint16_t red[G::NG][CK / 2];
 	const unsigned t = threadIdx.x % G::TPR;
 	const unsigned g = threadIdx.x / G::TPR;
@@ -193,35 +204,45 @@ __device__ uint32_t ck_reduce(const uint32_t* h, size_t nb, size_t run_end, cons
 	if(threadIdx.x < CK / 2) {
 		for(unsigned k = 0; k < G::NG; k++) acc ^= red[k][threadIdx.x];
 	}
+	__syncthreads();  // red is reused by the next slot
 	return acc;
 }
 
+// Persistent over slots: workgroup b prepares slots b, b + gridDim.x, ...
 template <unsigned CK>
-__global__ __launch_bounds__(kCkThreads) void k_prep(uint8_t* slots, const uint8_t* src_slots, size_t slot_bytes,
-                                                     const uint32_t* __restrict__ lens,
-                                                     const uint32_t* __restrict__ slot_index, size_t slice_size,
-                                                     size_t ck_off, int to_log, const uint16_t* __restrict__ lg,
-                                                     const uint16_t* __restrict__ ex) {
-	const unsigned k = blockIdx.x;
-	const size_t sidx = slot_index ? slot_index[k] : k;
-	uint8_t* slot = slots + sidx * slot_bytes;
-	const uint8_t* src = src_slots ? src_slots + sidx * slot_bytes : slot;
-	size_t len = lens ? lens[k] : slice_size;
-	if(len > slice_size) len = slice_size;
+__global__ __launch_bounds__(kPrepThreads) void k_prep(uint8_t* slots, const uint8_t* src_slots, size_t slot_bytes,
+                                                       unsigned count, const uint32_t* __restrict__ lens,
+                                                       const uint32_t* __restrict__ slot_index, size_t slice_size,
+                                                       size_t ck_off, int to_log, const uint16_t* __restrict__ lg,
+                                                       const uint16_t* __restrict__ ex) {
+	__shared__ uint16_t s_log[65536];
+	if(to_log) {
+		const uint4* a = reinterpret_cast<const uint4*>(lg);
+		uint4* b = reinterpret_cast<uint4*>(s_log);
+		for(unsigned k = threadIdx.x; k < 65536 * 2 / 16; k += kPrepThreads) b[k] = a[k];
+		__syncthreads();
+	}
 	const size_t nrows = slot_bytes / CK;
 	const size_t nb = (slice_size + CK - 1) / CK;
-	uint32_t h[CkGeom<CK>::DW];
-	size_t run_end;
-	if(to_log)
-		ck_rows<CK, 2>(src, slot, nrows, nb, len, h, lg, run_end);
-	else
-		ck_rows<CK, 1>(src, slot, nrows, nb, len, h, lg, run_end);
-	__threadfence();  // row rewrites complete before the checksum block is stored over them
-	uint32_t lane = ck_reduce<CK>(h, nb, run_end, lg, ex);
-	if(threadIdx.x < CK / 2) {
-		uint16_t v = (uint16_t)lane;
-		if(to_log) v = lg[v];
-		*reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = v;
+	for(unsigned k = blockIdx.x; k < count; k += gridDim.x) {
+		const size_t sidx = slot_index ? slot_index[k] : k;
+		uint8_t* slot = slots + sidx * slot_bytes;
+		const uint8_t* src = src_slots ? src_slots + sidx * slot_bytes : slot;
+		size_t len = lens ? lens[k] : slice_size;
+		if(len > slice_size) len = slice_size;
+		uint32_t h[CkGeom<CK, kPrepThreads>::DW];
+		size_t run_end;
+		if(to_log)
+			ck_rows<CK, kPrepThreads, 2>(src, slot, nrows, nb, len, h, s_log, run_end);
+		else
+			ck_rows<CK, kPrepThreads, 1>(src, slot, nrows, nb, len, h, s_log, run_end);
+		__threadfence();  // row rewrites complete before the checksum block is stored over them
+		uint32_t lane = ck_reduce<CK, kPrepThreads>(h, nb, run_end, lg, ex);
+		if(threadIdx.x < CK / 2) {
+			uint16_t v = (uint16_t)lane;
+			if(to_log) v = s_log[v];
+			*reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = v;
+		}
 	}
 }
 
@@ -233,10 +254,10 @@ __global__ __launch_bounds__(kCkThreads) void k_verify(const uint8_t* __restrict
 	const unsigned k = blockIdx.x;
 	uint8_t* slot = const_cast<uint8_t*>(slots) + (size_t)k * slot_bytes;
 	const size_t nb = (slice_size + CK - 1) / CK;
-	uint32_t h[CkGeom<CK>::DW];
+	uint32_t h[CkGeom<CK, kCkThreads>::DW];
 	size_t run_end;
-	ck_rows<CK, 0>(slot, slot, nb, nb, slice_size, h, lg, run_end);
-	uint32_t lane = ck_reduce<CK>(h, nb, run_end, lg, ex);
+	ck_rows<CK, kCkThreads, 0>(slot, slot, nb, nb, slice_size, h, lg, run_end);
+	uint32_t lane = ck_reduce<CK, kCkThreads>(h, nb, run_end, lg, ex);
 	__shared__ int bad;
 	if(threadIdx.x == 0) bad = 0;
 	__syncthreads();
@@ -254,12 +275,14 @@ hipError_t launch_prep(uint8_t* slots, const uint8_t* src, size_t slot_bytes, un
 	if(!count) return hipSuccess;
 	if(slot_bytes % kSlotAlign || ck_off + ck > slot_bytes || slice_size > ck_off || !valid_ck(ck))
 		return hipErrorInvalidValue;
-	dim3 grid(count), block(kCkThreads);
+	const unsigned blocks = count < (unsigned)(t.cus > 0 ? t.cus : 256) ? count : (unsigned)(t.cus > 0 ? t.cus : 256);
+	dim3 grid(blocks), block(kPrepThreads);
+	const int tl = to_log ? 1 : 0;
 	switch(ck) {
-		case 8: hipLaunchKernelGGL(k_prep<8>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
-		case 16: hipLaunchKernelGGL(k_prep<16>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
-		case 32: hipLaunchKernelGGL(k_prep<32>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
-		default: hipLaunchKernelGGL(k_prep<64>, grid, block, 0, s, slots, src, slot_bytes, lens, slot_index, slice_size, ck_off, to_log ? 1 : 0, t.log, t.exp); break;
+		case 8: hipLaunchKernelGGL(k_prep<8>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
+		case 16: hipLaunchKernelGGL(k_prep<16>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
+		case 32: hipLaunchKernelGGL(k_prep<32>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
+		default: hipLaunchKernelGGL(k_prep<64>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
 	}
 	return hipGetLastError();
 }
@@ -432,6 +455,20 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
 }
 
 // ------------------------------------------------------------------ misc
+
+// Plain streaming copy (16-byte vectors): the reference point for HBM traffic counters.
+__global__ void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+	for(size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+		dst[k] = src[k];
+}
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+	const size_t n = bytes / 16;
+	if(!n) return hipSuccess;
+	size_t blocks = (n + 255) / 256;
+	if(blocks > 8192) blocks = 8192;
+	hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n);
+	return hipGetLastError();
+}
 
 __global__ void k_xor_byte(uint8_t* p, uint8_t mask) {
 	if(threadIdx.x == 0 && blockIdx.x == 0) *p ^= mask;

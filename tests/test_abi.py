@@ -2,6 +2,7 @@
 declares, and its host-side helpers agree with the golden vectors.  No compute
 call needs a GPU here."""
 import ctypes
+import os
 import subprocess
 
 import numpy as np
@@ -89,3 +90,17 @@ def test_controller_allocation_rules():
     assert not c.init(100, [(Dummy(), 0, 51), (Dummy(), 51, 49)])
     assert not c.init(100, [(Dummy(), 0, 50), (Dummy(), 52, 48)])
     assert c.init(101, [(Dummy(), 0, 50), (Dummy(), 50, 51)])
+
+
+REF_INTEGRATION = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                               "ref_integration")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_INTEGRATION), reason="reference controller build not present")
+def test_par2prochip_links_with_reference_controller():
+    # ParPar's own PAR2Proc (built from the reference sources) with PAR2ProcHIP
+    # as its backend: the class compiles against controller.h and links
+    r = subprocess.run([REF_INTEGRATION, "--construct-only"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    assert "constructed" in r.stdout
