@@ -1,20 +1,26 @@
 """Benchmark: GF(2^16) recovery (out_r ^= sum_i c(r,i) * in_i, PAR2 coefficients)
 on one MI355X.
 
-Default workload: 2000 input slices x 1 MiB (uniform random, seed 1), 200
-recovery slices with exponents 0..199, everything resident in HBM.  One step is
-the whole device path over that batch: input preparation (zero padding,
-gf16_cksum, log-domain conversion), the plain mul-add, and the gf16_cksum check
-of every recovery slice.
+Workloads (--config):
+  1 (default)  2000 input slices x 1 MiB (uniform random, seed 1), 200 recovery
+               slices with exponents 0..199, plain form.
+  3            32768 input slices x 64 KiB (seed 3), 8192 recovery slices with
+               exponents 0..8191, dense form (int8 MFMA bit-matrix contraction),
+               with the plain form timed on the same data for the ratio.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+One step is the whole device path over the batch, everything resident in HBM:
+input preparation (zero padding, gf16_cksum, log domain for the plain form),
+the mul-add, and the gf16_cksum check of every recovery slice.
 
-Prints one JSON line.  value = GF16 mul-add GB/s of the whole step
-(2 bytes per mul-add, MA = inputs x recovery x words); roofline is the plain
-mul-add kernel's mul-add-equivalent bandwidth against HBM peak, timed with HIP
-events on its own stream; cpu_baseline times the oracle (the C restatement of
-PAR2ProcCPU) on a bounded column window of the same workload with every host
-thread.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|3] [--no-cpu]
+
+Prints one JSON line.  value = GF16 mul-add GB/s of the whole step (2 bytes
+per mul-add, MA = inputs x recovery x words).  roofline: the mul-add kernel,
+timed with HIP events on its stream -- the plain form against HBM peak
+(mul-add-equivalent bytes), the dense form against the int8 MFMA peak
+(512 int8 ops per mul-add).  cpu_baseline: the oracle (the C restatement of
+PAR2ProcCPU) with every host thread on a bounded sample of the same workload,
+compared byte for byte with the GPU's output.
 """
 import argparse
 import ctypes
@@ -27,11 +33,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
 
-# DRAM bytes per k_muladd_plain launch measured with rocprofv3 FETCH_SIZE /
-# WRITE_SIZE runs, scaled by a 1 GiB plain copy in the same runs
-# (profiles/k_muladd_plain_2000x1MiB_R200.md), keyed by (inputs, slice, recovery).
-MEASURED_TRAFFIC = {(2000, 1 << 20, 200): 19.21e9 + 0.210e9}
+# DRAM bytes per mul-add launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs, scaled by a 1 GiB plain copy in the same runs
+# (profiles/k_muladd_plain_2000x1MiB_R200.md), keyed by (form, inputs, slice, recovery).
+MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 19.21e9 + 0.210e9}
+
+CONFIGS = {
+    1: dict(inputs=2000, slice=1 << 20, recovery=200, seed=1, form="plain", steps=10, warmup=2),
+    3: dict(inputs=32768, slice=1 << 16, recovery=8192, seed=3, form="dense", steps=2, warmup=1),
+}
 
 
 def log(*a):
@@ -41,30 +53,41 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--inputs", type=int, default=2000)
-    ap.add_argument("--slice", type=int, default=1 << 20)
-    ap.add_argument("--recovery", type=int, default=200)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--inputs", type=int, default=None)
+    ap.add_argument("--slice", type=int, default=None)
+    ap.add_argument("--recovery", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--form", choices=["plain", "dense"], default=None)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="dense: skip timing the plain form beside it")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM counter reference)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    for k, v in CONFIGS[a.config].items():
+        if getattr(a, k) is None:
+            setattr(a, k, v)
+    return a
 
 
-def cpu_baseline(args, gpu_out_window, window_bytes_used):
-    """Oracle on the first `window` bytes of every slice (same inputs, same outputs)."""
+def cpu_threads():
+    t = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(t, 16))
+
+
+def cpu_baseline_window(args, gpu_out):
+    """Oracle on the first `window` bytes of every slice (all inputs, all recovery slices)."""
     import numpy as np
     import oracle as O
 
     n, R, S = args.inputs, args.recovery, args.slice
     exps = np.arange(R, dtype=np.uint16)
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    threads = cpu_threads()
 
     def run(win):
         inp = np.stack([O.prng_bytes(args.seed, 0, i, win) for i in range(n)])
@@ -77,23 +100,38 @@ def cpu_baseline(args, gpu_out_window, window_bytes_used):
     tile = 32768
     t, _ = run(tile)
     tiles = int(args.cpu_seconds * threads / max(t, 1e-9))
-    win = max(tile, min(tiles * tile, window_bytes_used, S))
+    win = max(tile, min(tiles * tile, S))
     win -= win % 1024
     log("cpu baseline: %d threads, window %d bytes of each slice" % (threads, win))
     t, out = run(win)
     ma = n * R * (win // 2)
-    check = None
-    if gpu_out_window is not None:
-        w = min(win, gpu_out_window.shape[1])
-        check = bool((out[:, :w] == gpu_out_window[:, :w]).all())
     return {
-        "value": round(2.0 * ma / t / 1e9, 3),
-        "unit": "GB/s",
-        "cores": threads,
-        "kind": "port",
+        "value": round(2.0 * ma / t / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
         "sample": "first %d bytes of all %d input slices x %d recovery slices (%.3g mul-adds, %.1f s)" % (
             win, n, R, ma, t),
-        "matches_gpu": check,
+        "matches_gpu": bool((out == gpu_out[:, :win]).all()),
+    }
+
+
+def cpu_baseline_rows(args, gpu_rows, rows):
+    """Oracle on a sample of recovery slices over the full inputs."""
+    import numpy as np
+    import oracle as O
+
+    n, S = args.inputs, args.slice
+    threads = cpu_threads()
+    inp = np.empty((n, S), np.uint8)
+    for i in range(n):
+        inp[i] = O.prng_bytes(args.seed, 0, i, S)
+    t0 = time.perf_counter()
+    out = O.recover(inp, [S] * n, S, np.array(rows, np.uint16), threads=threads)
+    t = time.perf_counter() - t0
+    ma = n * len(rows) * (S // 2)
+    return {
+        "value": round(2.0 * ma / t / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": "recovery slices %s over all %d input slices x %d B (%.3g mul-adds, %.1f s)" % (
+            rows, n, S, ma, t),
+        "matches_gpu": bool((out == gpu_rows).all()),
     }
 
 
@@ -112,10 +150,11 @@ def main():
     L = parpar_amd.lib()
     chk = parpar_amd.check
     n, R, S, ck = args.inputs, args.recovery, args.slice, args.cksum
+    dense = args.form == "dense"
     ck_off, slot = P.slot_layout(S, ck)
-    words = S // 2
-    ma = n * R * words
-    log("workload: %d inputs x %d bytes, %d recovery slices, MA=%.4g" % (n, S, R, ma))
+    ma = n * R * (S // 2)
+    log("workload: config %d, %s form, %d inputs x %d bytes, %d recovery slices, MA=%.4g" % (
+        args.config, args.form, n, S, R, ma))
 
     name = ctypes.create_string_buffer(256)
     cus = ctypes.c_int()
@@ -124,7 +163,7 @@ def main():
     stream = ctypes.c_void_p()
     chk(L.pph_stream_create(ctypes.byref(stream)))
     src = P.DeviceBuffer(slot * n)      # data-form inputs (arrive in HBM)
-    prepd = P.DeviceBuffer(slot * n)    # log-domain inputs with checksums
+    prepd = P.DeviceBuffer(slot * n)    # prepared inputs with checksums
     out = P.DeviceBuffer(slot * R)
     valid = P.DeviceBuffer(4 * R)
     d_logs = P.DeviceBuffer(2 * n)
@@ -135,25 +174,47 @@ def main():
     chk(L.pph_stream_synchronize(stream))
     log("inputs generated in HBM")
 
-    t_step = ctypes.c_void_p()
     t_ker = ctypes.c_void_p()
-    chk(L.pph_timer_create(ctypes.byref(t_step)))
     chk(L.pph_timer_create(ctypes.byref(t_ker)))
 
-    def step(timed):
-        chk(L.pph_prep_slots_from(prepd.ptr, src.ptr, slot, n, None, S, ck, 1, stream), "prep")
+    def muladd(form, timed):
         if timed:
             chk(L.pph_timer_start(t_ker, stream))
-        chk(L.pph_muladd_plain(prepd.ptr, slot, n, d_logs.ptr, d_exps.ptr, None, 0, out.ptr, R, 0, args.group,
-                               stream), "muladd")
+        if form == "dense":
+            chk(L.pph_muladd_dense(prepd.ptr, slot, n, d_logs.ptr, d_exps.ptr, out.ptr, R, 0, stream), "dense")
+        else:
+            chk(L.pph_muladd_plain(prepd.ptr, slot, n, d_logs.ptr, d_exps.ptr, None, 0, out.ptr, R, 0,
+                                   args.group, stream), "plain")
         if timed:
             chk(L.pph_timer_stop(t_ker, stream))
+
+    def step(form, timed):
+        chk(L.pph_prep_slots_from(prepd.ptr, src.ptr, slot, n, None, S, ck, 0 if form == "dense" else 1, stream),
+            "prep")
+        muladd(form, timed)
         chk(L.pph_verify_slots(out.ptr, slot, R, S, ck, valid.ptr, stream), "verify")
 
-    for w in range(args.warmup):
-        step(False)
+    def run_steps(form, steps, warmup):
+        for w in range(warmup):
+            step(form, False)
+            chk(L.pph_stream_synchronize(stream))
+            log("%s warmup %d done" % (form, w + 1))
+        ker = []
+        chk(L.pph_synchronize())
+        t0 = time.perf_counter()
+        for s in range(steps):
+            step(form, True)
+            ms = ctypes.c_float()
+            chk(L.pph_timer_elapsed_ms(t_ker, ctypes.byref(ms)))
+            ker.append(ms.value)
+            log("%s step %d: kernel %.2f ms" % (form, s + 1, ms.value))
         chk(L.pph_stream_synchronize(stream))
-        log("warmup %d done" % (w + 1))
+        wall = time.perf_counter() - t0
+        ok = valid.download(4 * R).view(np.int32)
+        if not (ok == 1).all():
+            raise SystemExit("recovery checksum mismatch on the device (%s): %d of %d slices invalid" % (
+                form, int((ok != 1).sum()), R))
+        return wall * 1e3 / steps, sum(ker) / len(ker)
 
     if args.calib_copy:
         cbytes = args.calib_copy - args.calib_copy % 16
@@ -164,44 +225,44 @@ def main():
         log("calibration copy of %d bytes done" % cbytes)
         del ca, cb
 
-    ker_ms = []
-    chk(L.pph_synchronize())
-    t0 = time.perf_counter()
-    chk(L.pph_timer_start(t_step, stream))
-    for s in range(args.steps):
-        step(True)
-        ms = ctypes.c_float()
-        chk(L.pph_timer_elapsed_ms(t_ker, ctypes.byref(ms)))
-        ker_ms.append(ms.value)
-    chk(L.pph_timer_stop(t_step, stream))
-    chk(L.pph_stream_synchronize(stream))
-    chk(L.pph_synchronize())
-    wall = time.perf_counter() - t0
-    total_ms = ctypes.c_float()
-    chk(L.pph_timer_elapsed_ms(t_step, ctypes.byref(total_ms)))
-    log("timed %d steps: %.2f ms wall, %.2f ms events" % (args.steps, wall * 1e3, total_ms.value))
+    compare = None
+    if dense and not args.no_compare:
+        _, p_ker = run_steps("plain", 1, 0)
+        compare = {"plain_kernel_ms": round(p_ker, 3), "plain_ma_per_s": round(ma / (p_ker / 1e3), 1)}
+        p_rows = out.download(min(R, 16) * slot).reshape(-1, slot)[:, :S].copy()
 
-    ok = valid.download(4 * R).view(np.int32)
-    if not (ok == 1).all():
-        raise SystemExit("recovery checksum mismatch on the device: %d of %d slices invalid" % (int((ok != 1).sum()), R))
-
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step, ker_avg = run_steps(args.form, args.steps, args.warmup)
     value = 2.0 * ma / (ms_per_step / 1e3) / 1e9
-    ker_avg = sum(ker_ms) / len(ker_ms)
-    achieved = 2.0 * ma / (ker_avg / 1e3) / 1e9
 
-    # window of the GPU outputs for the CPU cross-check
-    win = min(S, 1 << 20)
-    gpu_win = None
+    if dense:
+        achieved = 512.0 * ma / (ker_avg / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": MEASURED_TRAFFIC.get(("dense", n, S, R)),
+                "kernel": "k_muladd_dense", "kernel_ms": round(ker_avg, 3), "algorithmic_int8_ops": 512 * ma}
+        if compare is not None:
+            compare["dense_vs_plain"] = round(compare["plain_kernel_ms"] / ker_avg, 3)
+            d_rows = out.download(min(R, 16) * slot).reshape(-1, slot)[:, :S]
+            compare["dense_equals_plain_rows"] = bool((d_rows == p_rows).all())
+    else:
+        achieved = 2.0 * ma / (ker_avg / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(("plain", n, S, R)),
+                "kernel": "k_muladd_plain", "kernel_ms": round(ker_avg, 3), "algorithmic_bytes": 2 * ma}
+
     cpu = None
     if not args.no_cpu:
-        host = out.download(R * slot).reshape(R, slot)
-        gpu_win = host[:, :win]
-        cpu = cpu_baseline(args, gpu_win, win)
+        if args.config == 3:
+            rows = [0, 1, 2, 3, 4, 5, 6, 7, 1000, 2047, 4095, 5000, 6000, 7000, 8190, 8191]
+            rows = [r for r in rows if r < R]
+            got = np.stack([out.download(S, off=r * slot) for r in rows])
+            cpu = cpu_baseline_rows(args, got, rows)
+        else:
+            host = out.download(R * slot).reshape(R, slot)
+            cpu = cpu_baseline_window(args, host[:, :S])
         log("cpu baseline: %s" % cpu)
 
     res = {
-        "metric": "GF16 mul-add GB/s (2 B per GF(2^16) mul-add), plain form",
+        "metric": "GF16 mul-add GB/s (2 B per GF(2^16) mul-add), %s form" % args.form,
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -211,21 +272,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16 (GF(2^16) words)",
+        "dtype": "u16 (GF(2^16) words)" + (", int8 0/1 MFMA operands" if dense else ""),
         "data": "synthetic (counter-based random bytes, seed %d)" % args.seed,
-        "config": {"workload": "plain recovery: %d inputs x %d B slices, %d recovery slices (exp 0..%d)" % (
-            n, S, R, R - 1), "inputs": n, "slice_bytes": S, "recovery": R, "cksum_bytes": ck,
-            "mul_adds": ma, "device": name.value.decode(), "cus": cus.value},
+        "config": {"workload": "%s recovery: %d inputs x %d B slices, %d recovery slices (exp 0..%d)" % (
+            args.form, n, S, R, R - 1), "config_index": args.config, "inputs": n, "slice_bytes": S,
+            "recovery": R, "cksum_bytes": ck, "mul_adds": ma, "device": name.value.decode(), "cus": cus.value},
         "ma_per_s": round(ma / (ms_per_step / 1e3), 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((n, S, R)),
-                     "kernel": "k_muladd_plain", "kernel_ms": round(ker_avg, 3),
-                     "algorithmic_bytes": 2 * ma},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if compare is not None:
+        res["compare"] = compare
     print(json.dumps(res), flush=True)
-    L.pph_timer_destroy(t_step)
     L.pph_timer_destroy(t_ker)
     L.pph_stream_destroy(stream)
 

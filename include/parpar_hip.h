@@ -172,6 +172,12 @@ int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in,
                      const uint32_t* coef_log2, unsigned coef_stride,
                      void* out_slots, unsigned n_out, int accumulate, unsigned group, void* stream);
 
+/* Dense mul-add over data-form input slots (prepared with to_log = 0): the same
+ * result as pph_muladd_plain with PAR2 coefficients, computed as a GF(2)
+ * bit-matrix contraction on int8 MFMA with parity reduction. */
+int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                     const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
+
 /* gf16_cksum verification of n slots (copy_cksum_check, gf16/gf16_cksum_base.h:148-159):
  * valid[k] (device int array) = 1 if the stored block matches the slice data. */
 int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,

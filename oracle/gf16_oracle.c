@@ -197,7 +197,10 @@ int po_recover(unsigned nIn, const uint8_t* inputs, size_t inStride, const size_
 	memset(outputs, 0, (size_t)nOut * sliceSize);
 	if(!nIn || !nOut || !sliceSize) return 1;
 	const size_t tile = 32768; /* bytes of every slice per work item */
+	const unsigned outBlock = 8; /* recovery slices per work item */
 	size_t nTiles = (sliceSize + tile - 1) / tile;
+	size_t nBlocks = (nOut + outBlock - 1) / outBlock;
+	long nItems = (long)(nTiles * nBlocks);
 	int used = 1;
 #ifdef _OPENMP
 	if(threads <= 0) threads = omp_get_max_threads();
@@ -210,9 +213,12 @@ int po_recover(unsigned nIn, const uint8_t* inputs, size_t inStride, const size_
 #else
 	(void)threads;
 #endif
-	for(long t = 0; t < (long)nTiles; t++) {
+	for(long item = 0; item < nItems; item++) {
 		uint16_t lo[256], hi[256];
-		size_t off = (size_t)t * tile;
+		size_t t = (size_t)item / nBlocks;
+		unsigned r0 = (unsigned)((size_t)item % nBlocks) * outBlock;
+		unsigned r1 = r0 + outBlock < nOut ? r0 + outBlock : nOut;
+		size_t off = t * tile;
 		size_t len = sliceSize - off < tile ? sliceSize - off : tile;
 		for(unsigned i = 0; i < nIn; i++) {
 			size_t inLen = inLens ? inLens[i] : sliceSize;
@@ -220,7 +226,7 @@ int po_recover(unsigned nIn, const uint8_t* inputs, size_t inStride, const size_
 			size_t avail = inLen - off < len ? inLen - off : len;
 			const uint8_t* src = inputs + (size_t)i * inStride + off;
 			uint16_t logI = coeffs ? 0 : po_input_log(inputNums[i]);
-			for(unsigned r = 0; r < nOut; r++) {
+			for(unsigned r = r0; r < r1; r++) {
 				uint16_t c = coeffs ? coeffs[(size_t)i * nOut + r] : po_exp(po_coeff_log(logI, exps[r]));
 				if(!c) continue;
 				po_build_tables(c, lo, hi);

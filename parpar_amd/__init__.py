@@ -92,6 +92,7 @@ _SIGS = {
     "pph_prep_slots": (_i, [_vp, _sz, _u, _vp, _sz, _u, _i, _vp]),
     "pph_prep_slots_from": (_i, [_vp, _vp, _sz, _u, _vp, _sz, _u, _i, _vp]),
     "pph_muladd_plain": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _vp, _u, _i, _u, _vp]),
+    "pph_muladd_dense": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
 }
 

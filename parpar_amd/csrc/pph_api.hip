@@ -31,6 +31,7 @@ struct HostTables {
 	uint16_t exp[65536];
 	uint16_t log[65536];
 	uint16_t in_log[32768];
+	uint16_t rowmask[16 * 64];
 	HostTables() {
 		uint32_t n = 1;
 		for(uint32_t e = 0; e < kOrder; e++) {
@@ -48,6 +49,29 @@ struct HostTables {
 			do { e++; } while(!(e % 3) || !(e % 5) || !(e % 17) || !(e % 257));
 			in_log[i] = (uint16_t)e;
 		}
+		// bit-matrix rows: M_c[j][b] = bit j of (c * z^b) = XOR_k c_k bit j of z^(k+b);
+		// Q_j[k] = sum_b 2^b bit j of z^(k+b); T[j][q][n] = XOR of Q_j[4q+t] over bits t of n
+		uint32_t pw[31];
+		pw[0] = 1;
+		for(int m = 1; m < 31; m++) {
+			pw[m] = pw[m - 1] << 1;
+			if(pw[m] & 0x10000) pw[m] ^= kPoly;
+		}
+		for(int jj = 0; jj < 16; jj++) {
+			uint16_t Q[16];
+			for(int k = 0; k < 16; k++) {
+				uint32_t m = 0;
+				for(int b = 0; b < 16; b++) m |= ((pw[k + b] >> jj) & 1u) << b;
+				Q[k] = (uint16_t)m;
+			}
+			for(int qq = 0; qq < 4; qq++)
+				for(int nn = 0; nn < 16; nn++) {
+					uint16_t v = 0;
+					for(int t = 0; t < 4; t++)
+						if(nn & (1 << t)) v ^= Q[4 * qq + t];
+					rowmask[jj * 64 + qq * 16 + nn] = v;
+				}
+		}
 	}
 };
 static const HostTables& tables() {
@@ -56,6 +80,7 @@ static const HostTables& tables() {
 }
 const uint16_t* host_exp() { return tables().exp; }
 const uint16_t* host_log() { return tables().log; }
+const uint16_t* host_rowmask() { return tables().rowmask; }
 uint16_t host_input_log(uint16_t input) { return tables().in_log[input & 32767]; }
 uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent) {
 	uint32_t r = (uint32_t)input_log * exponent;
@@ -77,14 +102,17 @@ int device_tables(DeviceTables** out) {
 	DeviceTables& t = g_dev[dev];
 	if(!t.exp) {
 		const HostTables& h = tables();
-		uint16_t *e = nullptr, *l = nullptr;
+		uint16_t *e = nullptr, *l = nullptr, *rm = nullptr;
 		PPH_HIP(hipMalloc(&e, sizeof(h.exp)));
-		if(hipMalloc(&l, sizeof(h.log)) != hipSuccess) {
+		if(hipMalloc(&l, sizeof(h.log)) != hipSuccess || hipMalloc(&rm, sizeof(h.rowmask)) != hipSuccess) {
 			(void)hipFree(e);
+			if(l) (void)hipFree(l);
 			return PPH_E_NOMEM;
 		}
 		PPH_HIP(hipMemcpy(e, h.exp, sizeof(h.exp), hipMemcpyHostToDevice));
 		PPH_HIP(hipMemcpy(l, h.log, sizeof(h.log), hipMemcpyHostToDevice));
+		PPH_HIP(hipMemcpy(rm, h.rowmask, sizeof(h.rowmask), hipMemcpyHostToDevice));
+		t.rowmask = rm;
 		hipDeviceProp_t prop;
 		PPH_HIP(hipGetDeviceProperties(&prop, dev));
 		t.cus = prop.multiProcessorCount;
@@ -283,6 +311,19 @@ extern "C" int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigne
 	hipError_t e = launch_muladd_plain((const uint16_t*)in_slots, slot_bytes / 2, n_in, input_logs, exps, coef_log2,
 	                                   coef_stride, (uint16_t*)out_slots, n_out, accumulate != 0, group, *t,
 	                                   (hipStream_t)stream);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+extern "C" int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                                const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream) {
+	if((!in_slots && n_in) || !out_slots || slot_bytes % kSlotAlign) return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	hipError_t e = launch_muladd_dense((const uint16_t*)in_slots, slot_bytes / 2, n_in, input_logs, exps,
+	                                   (uint16_t*)out_slots, n_out, accumulate != 0, *t, (hipStream_t)stream);
 	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
 	if(e != hipSuccess) return set_error(e);
 	return PPH_OK;

@@ -28,12 +28,14 @@ constexpr unsigned kPlainWordsPerLane = 8;
 struct DeviceTables {
 	uint16_t* exp = nullptr;   // exp[k] = 2^k for k < 65535; exp[65535] = 0
 	uint16_t* log = nullptr;   // log[x] = log2(x) for x != 0; log[0] = 0xFFFF
+	uint16_t* rowmask = nullptr;  // [j][q][n]: row j of the bit-matrix of n * z^(4q), as a mask
 	int cus = 0;
 };
 
 // Host-side tables (built once).
 const uint16_t* host_exp();
 const uint16_t* host_log();
+const uint16_t* host_rowmask();  // 16 x 4 x 16 nibble tables of the coefficient bit-matrices
 uint16_t host_input_log(uint16_t input);
 uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent);
 
@@ -54,6 +56,9 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
                                const uint16_t* in_logs, const uint16_t* exps,
                                const uint32_t* coef, unsigned coef_stride,
                                uint16_t* out, unsigned n_out, bool accumulate, unsigned group,
+                               const DeviceTables& t, hipStream_t s);
+hipError_t launch_muladd_dense(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
+                               const uint16_t* exps, uint16_t* out, unsigned n_out, bool accumulate,
                                const DeviceTables& t, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -1,0 +1,118 @@
+"""Dense form parity: the int8-MFMA bit-matrix contraction against the oracle,
+the reference's golden vectors and the plain kernel.  Calls go through the C ABI."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import parpar_amd
+from parpar_amd import proc as P
+from golden_io import expected_valid, manifest
+
+pytestmark = pytest.mark.gpu
+G = manifest()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def lib():
+    return parpar_amd.lib()
+
+
+def run_dense(datas, lens, S, exps, nums=None, ck=64, split=None, also_plain=False):
+    n = len(datas)
+    _, slot = P.slot_layout(S, ck)
+    assert slot % 1024 == 0
+    host = np.full((max(n, 1), slot), 0xC3, np.uint8)
+    for k, d in enumerate(datas):
+        host[k, :len(d)] = d
+    src = P.DeviceBuffer(host.nbytes)
+    src.upload(host)
+    d_lens = P.DeviceBuffer(4 * max(n, 1))
+    d_lens.upload(np.array(lens or [0], np.uint32))
+    nums = list(range(n)) if nums is None else list(nums)
+    d_logs = P.DeviceBuffer(2 * max(n, 1))
+    d_logs.upload(np.array([lib().pph_input_log(x) for x in nums] or [0], np.uint16))
+    n_out = len(exps)
+    d_exps = P.DeviceBuffer(2 * n_out)
+    d_exps.upload(np.array(exps, np.uint16))
+    data = P.DeviceBuffer(host.nbytes)
+    parpar_amd.check(lib().pph_prep_slots_from(data.ptr, src.ptr, slot, n, d_lens.ptr, S, ck, 0, None), "prep")
+    out = P.DeviceBuffer(slot * n_out)
+    out.upload(np.full(slot * n_out, 0x77, np.uint8))
+    parts = [(0, n)] if split is None else [(0, split), (split, n)]
+    for bi, (a, b) in enumerate(parts):
+        parpar_amd.check(lib().pph_muladd_dense(data.offset(a * slot), slot, b - a, d_logs.offset(2 * a), d_exps.ptr,
+                                                out.ptr, n_out, 1 if bi else 0, None), "dense")
+    valid = P.DeviceBuffer(4 * n_out)
+    parpar_amd.check(lib().pph_verify_slots(out.ptr, slot, n_out, S, ck, valid.ptr, None), "verify")
+    parpar_amd.check(lib().pph_synchronize())
+    res = out.download().reshape(n_out, slot)
+    v = valid.download(4 * n_out).view(np.int32).copy()
+    plain = None
+    if also_plain:
+        logd = P.DeviceBuffer(host.nbytes)
+        parpar_amd.check(lib().pph_prep_slots_from(logd.ptr, src.ptr, slot, n, d_lens.ptr, S, ck, 1, None))
+        pout = P.DeviceBuffer(slot * n_out)
+        parpar_amd.check(lib().pph_muladd_plain(logd.ptr, slot, n, d_logs.ptr, d_exps.ptr, None, 0, pout.ptr, n_out,
+                                                0, 0, None))
+        parpar_amd.check(lib().pph_synchronize())
+        plain = pout.download().reshape(n_out, slot)
+    return res, v, plain
+
+
+def test_dense_single_input_single_slice():
+    say("dense 1x1")
+    S = 2048
+    d = O.prng_bytes(70, 0, 0, S)
+    got, valid, _ = run_dense([d], [S], S, [1])
+    ref = O.recover(d[None, :], [S], S, [1])
+    assert (got[0, :S] == ref[0]).all()
+    assert valid.all()
+
+
+@pytest.mark.parametrize("n,n_out,S", [(1, 1, 1024), (2, 8, 4096), (3, 9, 4000), (17, 13, 8192), (64, 40, 20000),
+                                       (5, 3, 1)])
+def test_dense_vs_oracle(n, n_out, S):
+    say("dense", n, n_out, S)
+    rng = np.random.default_rng(n * 1000 + n_out)
+    lens = [S] * n
+    lens[-1] = int(rng.integers(0, S + 1))
+    datas = [O.prng_bytes(71, n, k, lens[k]) for k in range(n)]
+    if lens[0] > 100:
+        datas[0][20:60] = 0
+    exps = rng.integers(0, 65535, n_out).astype(np.uint16)
+    exps[0] = 0
+    nums = rng.integers(0, 32768, n).astype(np.uint16)
+    got, valid, plain = run_dense(datas, lens, S, exps, nums=nums, split=n // 2 if n > 2 else None,
+                                  also_plain=True)
+    inp = np.zeros((n, max(S, 1)), np.uint8)
+    for k in range(n):
+        inp[k, :lens[k]] = datas[k]
+    ref = O.recover(inp, lens, S, exps, input_nums=nums)
+    assert (got[:, :S] == ref).all()
+    assert ((valid == 1) == expected_valid(inp, lens, S, exps, nums)).all()
+    # the dense and plain forms agree on the whole slot, checksum block included
+    assert (got == plain).all()
+
+
+def test_dense_config0_golden():
+    say("dense config0")
+    c = G["config0"]
+    S, n = c["slice"], c["n"]
+    datas = [O.prng_bytes(0, 0, i, S) for i in range(n)]
+    got, valid, _ = run_dense(datas, [S] * n, S, c["exps"])
+    off, _ = P.slot_layout(S, 64)
+    for r in range(len(c["exps"])):
+        assert hashlib.sha256(got[r, :S].tobytes()).hexdigest() == c["sha256"][r]
+        assert bytes(got[r, off:off + 64]).hex() == c["trailer"][r]
+    assert (valid == 1).all()
+
+
+def test_dense_rejects_bad_shapes():
+    buf = P.DeviceBuffer(4096)
+    assert lib().pph_muladd_dense(buf.ptr, 1000, 1, buf.ptr, buf.ptr, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
+    assert lib().pph_muladd_dense(buf.ptr, 1024, 1, None, None, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
