@@ -7,13 +7,13 @@
 // w, on 0/1 int8 operands: v_mfma_i32_32x32x32_i8 accumulates exact int32 sums
 // (at most 16 * inputs <= 2^19) and the low bit is the GF(2) result.
 //
-// Workgroup: 4 waves, 8 recovery slices x 512 words.  Wave q owns words
-// [128q, 128q+128) of the block as 4 x 4 MFMA tiles of 32 x 32 (256
+// Workgroup: 8 waves (two per SIMD), 8 recovery slices x 512 words.  Wave q
+// owns words [64q, 64q+64) of the block as 4 x 2 MFMA tiles of 32 x 32 (128
 // accumulators).  A K-step covers 2 inputs (K = 32).  The A fragments (rows of
-// the coefficient bit-matrices) are the same for the 4 waves: wave q builds
-// fragment q for the next step into LDS (double buffered, one barrier per
-// step); every wave expands its own B fragments (16 bits of one input word per
-// lane -> 16 bytes).
+// the coefficient bit-matrices) are the same for all waves: waves q and q+4
+// build fragment q for the next chunk into LDS (double buffered, one barrier per
+// chunk of steps); every wave expands its own B fragments (16 bits of one
+// input word per lane -> 16 bytes).
 //
 // Row j of M_c, as a 16-bit mask over b, is XOR_k c_k Q_j[k] with
 // Q_j[k] = sum_b 2^b * bit j of z^(k+b) mod p; it is read from four
@@ -25,9 +25,12 @@ namespace pph {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr unsigned kDenseThreads = 256;
+constexpr unsigned kDenseWaves = 8;    // two waves per SIMD: one's VALU overlaps the other's MFMAs
+constexpr unsigned kDenseNT = 2;       // 32-word MFMA column tiles per wave
+constexpr unsigned kDenseThreads = 64 * kDenseWaves;
 constexpr unsigned kDenseRows = 8;     // recovery slices per workgroup
-constexpr unsigned kDenseCols = 512;   // words per workgroup
+constexpr unsigned kDenseCols = kDenseWaves * 32 * kDenseNT;  // words per workgroup (512)
+constexpr unsigned kRmStride = 66;     // LDS row stride of the rowmask tables (odd dword count: no bank aliasing)
 
 // bit b of x (b < 16) -> byte b, as 0 or 1
 __device__ __forceinline__ v4i expand16(uint32_t x) {
@@ -50,14 +53,24 @@ __device__ __forceinline__ uint32_t rowmask(const uint16_t* t_j, uint32_t c) {
 	       (uint32_t)t_j[48 + (c >> 12)];
 }
 
+constexpr unsigned kDenseChunk = 4;        // K-steps (2 inputs each) per pipeline chunk
+constexpr unsigned kDenseMaxInputs = 32768;  // PAR2 input limit; logs resident in LDS
+
+// The K loop runs in chunks of kDenseChunk steps.  While the MFMAs of chunk k
+// run, the wave has already issued the word loads of chunk k+1 and the
+// coefficient gathers of chunk k+2, and builds the A fragments of chunk k+1 from
+// coefficients gathered one iteration earlier: global latency stays hidden
+// behind 128 MFMAs, and there is one barrier per chunk.
 __global__ __launch_bounds__(kDenseThreads) void k_muladd_dense(
     const uint16_t* __restrict__ in, size_t slot_words, unsigned n_in, const uint16_t* __restrict__ in_logs,
     const uint16_t* __restrict__ exps, uint16_t* __restrict__ out, unsigned n_out, int accumulate,
     const uint16_t* __restrict__ exp_tab, const uint16_t* __restrict__ rm_tab, unsigned row_blocks,
     unsigned col_blocks) {
-	__shared__ uint16_t s_rm[16 * 64];
-	__shared__ v4i s_a[2][4][64];
-	for(unsigned k = threadIdx.x; k < 16 * 64; k += kDenseThreads) s_rm[k] = rm_tab[k];
+	__shared__ uint16_t s_rm[16 * kRmStride];
+	__shared__ uint16_t s_logs[kDenseMaxInputs];
+	__shared__ v4i s_a[2][kDenseChunk][4][64];
+	for(unsigned k = threadIdx.x; k < 16 * 64; k += kDenseThreads) s_rm[(k >> 6) * kRmStride + (k & 63)] = rm_tab[k];
+	for(unsigned k = threadIdx.x; k < n_in; k += kDenseThreads) s_logs[k] = in_logs[k];
 
 	// blocks of one column block run on the same XCD (blockIdx round-robins over 8 XCDs)
 	const unsigned total = row_blocks * col_blocks;
@@ -71,68 +84,92 @@ __global__ __launch_bounds__(kDenseThreads) void k_muladd_dense(
 	const unsigned n = lane & 31u;           // MFMA column / A row within the tile
 	const unsigned g = lane >> 5;            // K group: which of the step's two inputs
 	const unsigned r_base = rb * kDenseRows;
-	const size_t w_base = (size_t)cb * kDenseCols + q * 128;
+	const size_t w_base = (size_t)cb * kDenseCols + q * (32 * kDenseNT);
 
-	// the A row this lane builds: fragment q, slice r_base + 2q + n/16, bit j = n%16
-	const unsigned r_a = r_base + 2 * q + (n >> 4);
+	// the A row this lane builds: fragment q%4, slice r_base + 2(q%4) + n/16, bit
+	// j = n%16; waves q and q+4 build the even / odd steps of each chunk
+	const unsigned fq = q & 3u;
+	const unsigned t_par = q >> 2;
+	const unsigned r_a = r_base + 2 * fq + (n >> 4);
 	const unsigned j = n & 15u;
 	const bool row_ok = r_a < n_out;
 	const uint32_t e_a = row_ok ? exps[r_a] : 0u;
-	const uint16_t* t_j = s_rm + j * 64;
+	const uint16_t* t_j = s_rm + j * kRmStride;
 	__syncthreads();
 
-	v16i acc[4][4];
+	v16i acc[4][kDenseNT];
 #pragma unroll
 	for(int a = 0; a < 4; a++)
 #pragma unroll
-		for(int b = 0; b < 4; b++)
+		for(int b = 0; b < (int)kDenseNT; b++)
 #pragma unroll
 			for(int v = 0; v < 16; v++) acc[a][b][v] = 0;
 
 	const unsigned steps = (n_in + 1) / 2;
-	// software pipeline: coefficient and input words of the next step are
-	// fetched while the current step runs
-	auto coef_of = [&](unsigned i) -> uint32_t {
-		if(i >= n_in || !row_ok) return 0u;
-		return exp_tab[mod65535d((uint32_t)in_logs[i] * e_a)];
-	};
-	auto words_of = [&](unsigned i) -> uint2 {
-		if(i >= n_in) return make_uint2(0, 0);
-		return *reinterpret_cast<const uint2*>(in + (size_t)i * slot_words + w_base + 4 * n);
-	};
-	uint32_t c_next = coef_of(g);
-	uint2 w_next = words_of(g);
-	for(unsigned s = 0; s < steps; s++) {
-		const uint32_t c_cur = c_next;
-		const uint2 w_cur = w_next;
-		const unsigned i_next = 2 * (s + 1) + g;
-		c_next = coef_of(i_next);
-		w_next = words_of(i_next);
+	const unsigned chunks = (steps + kDenseChunk - 1) / kDenseChunk;
+	const uint16_t* in_lane = in + w_base + kDenseNT * n;
+	constexpr unsigned kBuild = kDenseChunk / 2;  // steps of a chunk this wave builds A for
 
-		s_a[s & 1][q][lane] = expand16(rowmask(t_j, c_cur));
+	uint32_t cv_next[kBuild], cv_next2[kBuild];
+	uint32_t w_cur[kDenseChunk], w_next[kDenseChunk];
+	auto gather = [&](unsigned chunk, uint32_t* cv) {
+#pragma unroll
+		for(unsigned u = 0; u < kBuild; u++) {
+			const unsigned i = 2 * (chunk * kDenseChunk + 2 * u + t_par) + g;
+			cv[u] = (i < n_in && row_ok) ? (uint32_t)exp_tab[mod65535d((uint32_t)s_logs[i] * e_a)] : 0u;
+		}
+	};
+	auto load_words = [&](unsigned chunk, uint32_t* w) {
+#pragma unroll
+		for(unsigned t = 0; t < kDenseChunk; t++) {
+			const unsigned i = 2 * (chunk * kDenseChunk + t) + g;
+			w[t] = i < n_in ? *reinterpret_cast<const uint32_t*>(in_lane + (size_t)i * slot_words) : 0u;
+		}
+	};
+	auto build_a = [&](unsigned chunk, const uint32_t* cv) {
+#pragma unroll
+		for(unsigned u = 0; u < kBuild; u++)
+			s_a[chunk & 1][2 * u + t_par][fq][lane] = expand16(rowmask(t_j, cv[u]));
+	};
+
+	gather(0, cv_next);
+	build_a(0, cv_next);
+	gather(1, cv_next);
+	load_words(0, w_cur);
+	__syncthreads();
+	for(unsigned k = 0; k < chunks; k++) {
+		gather(k + 2, cv_next2);
+		load_words(k + 1, w_next);
+		if(k + 1 < chunks) build_a(k + 1, cv_next);
+#pragma unroll
+		for(unsigned t = 0; t < kDenseChunk; t++) {
+			v4i a[4], b[kDenseNT];
+#pragma unroll
+			for(int f = 0; f < 4; f++) a[f] = s_a[k & 1][t][f][lane];
+			b[0] = expand16(w_cur[t] & 0xFFFFu);
+			b[1] = expand16(w_cur[t] >> 16);
+#pragma unroll
+			for(int fa = 0; fa < 4; fa++)
+#pragma unroll
+				for(int fb = 0; fb < (int)kDenseNT; fb++)
+					acc[fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[fa], b[fb], acc[fa][fb], 0, 0, 0);
+		}
 		__syncthreads();
-		v4i a[4], b[4];
 #pragma unroll
-		for(int f = 0; f < 4; f++) a[f] = s_a[s & 1][f][lane];
-		b[0] = expand16(w_cur.x & 0xFFFFu);
-		b[1] = expand16(w_cur.x >> 16);
-		b[2] = expand16(w_cur.y & 0xFFFFu);
-		b[3] = expand16(w_cur.y >> 16);
+		for(unsigned t = 0; t < kDenseChunk; t++) w_cur[t] = w_next[t];
 #pragma unroll
-		for(int fa = 0; fa < 4; fa++)
-#pragma unroll
-			for(int fb = 0; fb < 4; fb++)
-				acc[fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[fa], b[fb], acc[fa][fb], 0, 0, 0);
+		for(unsigned u = 0; u < kBuild; u++) cv_next[u] = cv_next2[u];
 	}
 
 	// parity -> words.  32x32 i32 result: lane l, element v holds row
 	// 8*(v/4) + 4*(l/32) + v%4, column l%32.  Rows 16h..16h+15 are the 16 bits of
 	// slice 2*fa + h; this lane holds 8 of them, its partner lane l^32 the rest.
+	// Column n of tile fb is word w_base + NT*n + fb.
 #pragma unroll
 	for(int fa = 0; fa < 4; fa++) {
-		uint32_t word[4][2];
+		uint32_t word[kDenseNT][2];
 #pragma unroll
-		for(int fb = 0; fb < 4; fb++) {
+		for(int fb = 0; fb < (int)kDenseNT; fb++) {
 #pragma unroll
 			for(int h = 0; h < 2; h++) {
 				uint32_t bits = 0;
@@ -144,19 +181,12 @@ __global__ __launch_bounds__(kDenseThreads) void k_muladd_dense(
 				word[fb][h] = bits | (uint32_t)__shfl_xor((int)bits, 32);
 			}
 		}
-		// lane group g writes slice 2*fa + g: words w_base + 4n .. +3
+		// lane group g writes slice 2*fa + g: words w_base + NT*n .. +NT-1
 		const unsigned r = r_base + 2 * fa + g;
 		if(r < n_out) {
-			uint2 o;
-			o.x = (g ? word[0][1] : word[0][0]) | ((g ? word[1][1] : word[1][0]) << 16);
-			o.y = (g ? word[2][1] : word[2][0]) | ((g ? word[3][1] : word[3][0]) << 16);
-			uint2* dst = reinterpret_cast<uint2*>(out + (size_t)r * slot_words + w_base + 4 * n);
-			if(accumulate) {
-				uint2 prev = *dst;
-				o.x ^= prev.x;
-				o.y ^= prev.y;
-			}
-			*dst = o;
+			const uint32_t o = (g ? word[0][1] : word[0][0]) | ((g ? word[1][1] : word[1][0]) << 16);
+			uint32_t* dst = reinterpret_cast<uint32_t*>(out + (size_t)r * slot_words + w_base + kDenseNT * n);
+			*dst = accumulate ? (o ^ *dst) : o;
 		}
 	}
 }
@@ -167,6 +197,7 @@ hipError_t launch_muladd_dense(const uint16_t* in, size_t slot_words, unsigned n
 	if(!n_out || !slot_words) return hipSuccess;
 	if(slot_words % kDenseCols) return hipErrorInvalidValue;
 	if(n_in && (!in_logs || !exps || !in)) return hipErrorInvalidValue;
+	if(n_in > kDenseMaxInputs) return hipErrorInvalidValue;
 	if(!t.rowmask) return hipErrorInvalidValue;
 	const unsigned row_blocks = (n_out + kDenseRows - 1) / kDenseRows;
 	const unsigned col_blocks = (unsigned)(slot_words / kDenseCols);
