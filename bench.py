@@ -4,6 +4,11 @@ on one MI355X.
 Workloads (--config):
   1 (default)  2000 input slices x 1 MiB (uniform random, seed 1), 200 recovery
                slices with exponents 0..199, plain form.
+  2            64 independent jobs (seed 2): inputs U[10,3000], slice size
+               log-uniform in [64 KiB, 4 MiB] (multiple of 4), last input
+               truncated to a random length, recovery count U[1,500], all in one
+               persistent fused launch replayed from a hipGraph, forms and
+               grouping chosen by the autotuner.
   3            32768 input slices x 64 KiB (seed 3), 8192 recovery slices with
                exponents 0..8191, dense form (int8 MFMA bit-matrix contraction),
                with the plain form timed on the same data for the ratio.
@@ -12,7 +17,7 @@ One step is the whole device path over the batch, everything resident in HBM:
 input preparation (zero padding, gf16_cksum, log domain for the plain form),
 the mul-add, and the gf16_cksum check of every recovery slice.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3] [--no-cpu]
 
 Prints one JSON line.  value = GF16 mul-add GB/s of the whole step (2 bytes
 per mul-add, MA = inputs x recovery x words).  roofline: the mul-add kernel,
@@ -42,8 +47,139 @@ MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 19.21e9 + 0.210e9}
 
 CONFIGS = {
     1: dict(inputs=2000, slice=1 << 20, recovery=200, seed=1, form="plain", steps=10, warmup=2),
+    2: dict(inputs=None, slice=None, recovery=None, seed=2, form="batched", steps=2, warmup=1),
     3: dict(inputs=32768, slice=1 << 16, recovery=8192, seed=3, form="dense", steps=2, warmup=1),
 }
+
+
+def batched_jobs(seed, count=64):
+    """The batched workload's job shapes: (inputs, slice bytes, last input bytes, recovery)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(count):
+        n = int(rng.integers(10, 3001))
+        s = int(np.exp(rng.uniform(np.log(64 << 10), np.log(4 << 20))))
+        s -= s % 4
+        last = int(rng.integers(1, s + 1))
+        r = int(rng.integers(1, 501))
+        jobs.append((n, s, last, r))
+    return jobs
+
+
+def main_batched(args):
+    import numpy as np
+    import parpar_amd
+    from parpar_amd import batch as B
+
+    parpar_amd.require_device()
+    L = parpar_amd.lib()
+    chk = parpar_amd.check
+    shapes = batched_jobs(args.seed)
+    jobs = []
+    ma = 0
+    for j, (n, S, last, R) in enumerate(shapes):
+        lens = [S] * n
+        lens[-1] = last
+        jobs.append(B.Job(n, S, np.arange(R), lens=lens, cksum=args.cksum))
+        ma += n * R * (S // 2)
+    in_bytes = sum(j.slot * j.n_in for j in jobs)
+    out_bytes = sum(j.slot * j.n_out for j in jobs)
+    log("batched workload: %d jobs, %.1f GiB inputs, %.1f GiB outputs, MA=%.4g" % (
+        len(jobs), in_bytes / 2**30, out_bytes / 2**30, ma))
+    name = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    L.pph_device_info(0, name, 256, ctypes.byref(cus))
+    stream = ctypes.c_void_p()
+    chk(L.pph_stream_create(ctypes.byref(stream)))
+
+    t0 = time.perf_counter()
+    tuning = B.autotune()
+    log("autotune: %.2f s %s" % (time.perf_counter() - t0, tuning.as_dict()))
+    batch = B.Batch(jobs, cksum=args.cksum, tuning=tuning)
+    forms = [batch.plan(j)["form"] for j in range(len(jobs))]
+    group = batch.plan(0)["group"]
+    log("plan: group %d, forms %s" % (group, {f: forms.count(f) for f in set(forms)}))
+
+    def refill():
+        for j, job in enumerate(jobs):
+            job.fill(args.seed, j, stream)
+        chk(L.pph_stream_synchronize(stream))
+
+    # eager run: fused kernel time from HIP events
+    refill()
+    batch.run(use_graph=False, stream=stream)
+    fused_ms = batch.fused_ms()
+    chk(L.pph_stream_synchronize(stream))
+    log("eager run: fused mul-add %.2f ms" % fused_ms)
+    for w in range(args.warmup):
+        refill()
+        batch.run(use_graph=True, stream=stream)
+        chk(L.pph_stream_synchronize(stream))
+        log("graph warmup %d done" % (w + 1))
+    times = []
+    for s_ in range(args.steps):
+        refill()
+        chk(L.pph_synchronize())
+        t0 = time.perf_counter()
+        batch.run(use_graph=True, stream=stream)
+        chk(L.pph_stream_synchronize(stream))
+        times.append(time.perf_counter() - t0)
+        log("graph step %d: %.2f ms" % (s_ + 1, times[-1] * 1e3))
+    bad = sum(int((job.valid() != 1).sum()) for job in jobs if job.slice_size % 2 == 0)
+    if bad:
+        raise SystemExit("batched: %d recovery slices failed the device checksum" % bad)
+    ms_per_step = 1e3 * sum(times) / len(times)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = cpu_threads()
+        rng = np.random.default_rng(1000 + args.seed)
+        ok = True
+        cma = 0
+        t_cpu = 0.0
+        for j, job in enumerate(jobs):
+            win = min(4096, job.slice_size)
+            off = int(rng.integers(0, job.slice_size - win + 1)) & ~1
+            lens = [max(0, min(win, int(job.lens[k]) - off)) for k in range(job.n_in)]
+            inp = np.zeros((job.n_in, win), np.uint8)
+            for k in range(job.n_in):
+                if lens[k]:
+                    inp[k, :lens[k]] = O.prng_bytes(args.seed, j, k, lens[k], offset=off)
+            t0 = time.perf_counter()
+            ref = O.recover(inp, lens, win, job.exps, threads=threads)
+            t_cpu += time.perf_counter() - t0
+            cma += job.n_in * job.n_out * (win // 2)
+            got = np.stack([job.d_out.download(win, off=r * job.slot + off) for r in range(job.n_out)])
+            ok = ok and bool((got == ref).all())
+        cpu = {"value": round(2.0 * cma / t_cpu / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": "a 4 KiB column window of every job (all inputs, all recovery slices; %.3g mul-adds, "
+                         "%.1f s)" % (cma, t_cpu), "matches_gpu": ok}
+        log("cpu baseline: %s" % cpu)
+
+    achieved = 2.0 * ma / (fused_ms / 1e3) / 1e9
+    res = {
+        "metric": "GF16 mul-add GB/s (2 B per GF(2^16) mul-add), batched uneven jobs",
+        "value": round(2.0 * ma / (ms_per_step / 1e3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16 (GF(2^16) words)",
+        "data": "synthetic (counter-based random bytes, seed %d)" % args.seed,
+        "config": {"workload": "batched: 64 uneven PAR2 jobs in one persistent launch (hipGraph)", "config_index": 2,
+                   "jobs": len(jobs), "mul_adds": ma, "input_bytes": in_bytes, "output_bytes": out_bytes,
+                   "plain_group": group, "forms": {f: forms.count(f) for f in set(forms)},
+                   "device": name.value.decode(), "cus": cus.value},
+        "ma_per_s": round(ma / (ms_per_step / 1e3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_muladd_batch (+ dense launches)", "kernel_ms": round(fused_ms, 3),
+                     "algorithmic_bytes": 2 * ma},
+        "cpu_baseline": cpu,
+        "autotune": tuning.as_dict(),
+    }
+    print(json.dumps(res), flush=True)
+    batch.close()
+    L.pph_stream_destroy(stream)
 
 
 def log(*a):
@@ -60,7 +196,7 @@ def parse():
     ap.add_argument("--slice", type=int, default=None)
     ap.add_argument("--recovery", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--form", choices=["plain", "dense"], default=None)
+    ap.add_argument("--form", choices=["plain", "dense", "batched"], default=None)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
@@ -141,6 +277,9 @@ def main():
         print("bench.py measures one GPU: this engine runs on a single device (byte-range sharding across "
               "GPUs is a design note only); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == 2:
+        return main_batched(args)
 
     import numpy as np
     import parpar_amd

@@ -183,6 +183,53 @@ int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, con
 int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                      unsigned cksum_size, int* valid, void* stream);
 
+/* ======================================================================
+ * Batched execution: many independent jobs in one persistent launch
+ * ====================================================================== */
+/* One PAR2 job, device-resident.  Inputs are data-form slots on entry to every
+ * run and are prepared in place (zero padding, gf16_cksum, log domain for the
+ * plain form), so the caller refills them before the next run. */
+typedef struct pph_job {
+	void* in_slots;            /* n_in input slots of slot_bytes */
+	void* out_slots;           /* n_out recovery slots of slot_bytes */
+	const uint32_t* in_lens;   /* device: bytes of each input (<= slice_size); NULL = slice_size */
+	const uint16_t* in_logs;   /* device: PAR2 input logs (pph_input_log of the input numbers) */
+	const uint16_t* exps;      /* device: recovery exponents */
+	int* valid;                /* device: n_out gf16_cksum flags, written by every run */
+	size_t slice_size;
+	size_t slot_bytes;         /* >= pph_slot_layout(slice_size, cksum_size) */
+	unsigned n_in, n_out;      /* 1 <= n_in <= 32768 when n_out > 0 */
+} pph_job;
+
+/* Measured mul-add rates of the forms by recovery-count bucket
+ * (R <= 1, 4, 16, 64, 256, 1024, 4096, more), mul-adds per second. */
+typedef struct pph_tuning {
+	unsigned version;
+	unsigned r_buckets;
+	float plain_g8[8];
+	float plain_g16[8];
+	float dense[8];
+} pph_tuning;
+
+/* Measure the rates on the current device (about a second of GPU time). */
+int pph_autotune(pph_tuning* out);
+
+typedef struct pph_batch pph_batch;
+/* Plan a batch: tuning NULL runs (once per process) and uses pph_autotune. */
+int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned cksum_size, const pph_tuning* tuning,
+                     pph_batch** out);
+/* The form (PPH_FORM_PLAIN / PPH_FORM_DENSE) chosen for a job, the plain
+ * kernel's output grouping and the predicted mul-add time of the batch. */
+int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group, double* predicted_s);
+/* Run the batch on a stream: counter reset, per-job preparation, the fused
+ * persistent mul-add (and dense launches), per-job verification.  use_graph
+ * captures that sequence into a hipGraph on first use and replays it. */
+int pph_batch_run(pph_batch* b, int use_graph, void* stream);
+/* Time of the mul-add launches of the last eager (use_graph = 0) run, from HIP
+ * events recorded on its stream around them; waits for that run. */
+int pph_batch_fused_ms(pph_batch* b, float* ms);
+void pph_batch_destroy(pph_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

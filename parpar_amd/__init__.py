@@ -93,6 +93,12 @@ _SIGS = {
     "pph_prep_slots_from": (_i, [_vp, _vp, _sz, _u, _vp, _sz, _u, _i, _vp]),
     "pph_muladd_plain": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _vp, _u, _i, _u, _vp]),
     "pph_muladd_dense": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
+    "pph_autotune": (_i, [_vp]),
+    "pph_batch_create": (_i, [_vp, _u, _u, _vp, ctypes.POINTER(_vp)]),
+    "pph_batch_plan": (_i, [_vp, _u, ctypes.POINTER(_i), ctypes.POINTER(_u), ctypes.POINTER(ctypes.c_double)]),
+    "pph_batch_run": (_i, [_vp, _i, _vp]),
+    "pph_batch_destroy": (None, [_vp]),
+    "pph_batch_fused_ms": (_i, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
 }
 

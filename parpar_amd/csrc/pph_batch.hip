@@ -1,0 +1,394 @@
+// Batched execution: many independent PAR2 jobs (uneven input counts, slice
+// sizes and recovery counts) in one persistent fused mul-add launch, with the
+// per-job preparation and checksum verification around it captured once in a
+// hipGraph and replayed.  The form (plain or dense) and the plain kernel's
+// output grouping are chosen per batch from rates the autotuner measures on
+// the device.
+#include "pph_internal.h"
+#include "pph_plain_tile.h"
+#include "../../include/parpar_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace pph {
+
+struct BatchJob {
+	const uint16_t* in;
+	uint16_t* out;
+	const uint16_t* in_logs;
+	const uint16_t* exps;
+	uint64_t slot_words;
+	uint32_t n_in, n_out;
+};
+
+struct BatchItem {
+	uint32_t job;
+	uint32_t grp;
+	uint32_t col;
+	uint32_t pad;
+};
+
+// Persistent: one workgroup per CU loads the exp table once, then takes work
+// items (job, column tile, output group) from an atomic counter until they run
+// out.  Every wave leaves the loop on the same (LDS-broadcast) item index.
+template <int G>
+__global__ __launch_bounds__(kPlainThreads) void k_muladd_batch(const BatchJob* __restrict__ jobs,
+                                                                const BatchItem* __restrict__ items, unsigned n_items,
+                                                                unsigned* __restrict__ counter,
+                                                                const uint16_t* __restrict__ exp_tab) {
+	__shared__ uint16_t s_exp[65536];
+	__shared__ unsigned s_item;
+	load_exp_table(s_exp, exp_tab);
+	for(;;) {
+		if(threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+		__syncthreads();
+		const unsigned k = s_item;
+		__syncthreads();  // every thread has read s_item before it is overwritten
+		if(k >= n_items) break;
+		const BatchItem it = items[k];
+		const BatchJob jb = jobs[it.job];
+		plain_tile<G, false>(s_exp, jb.in, jb.slot_words, jb.n_in, jb.in_logs, jb.exps, nullptr, 0, jb.out, jb.n_out,
+		                     0, it.col, it.grp);
+	}
+}
+
+__global__ void k_set_u32(unsigned* p, unsigned v) {
+	if(threadIdx.x == 0) *p = v;
+}
+
+// ------------------------------------------------------------------ autotuner
+
+constexpr unsigned kTuneBuckets = 8;
+static const unsigned kTuneR[kTuneBuckets] = {1, 4, 16, 64, 256, 1024, 4096, 8192};
+
+unsigned tune_bucket(unsigned r) {
+	for(unsigned b = 0; b < kTuneBuckets; b++)
+		if(r <= kTuneR[b]) return b;
+	return kTuneBuckets - 1;
+}
+
+int autotune(pph_tuning* out) {
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	const unsigned N = 512;
+	const size_t S = 65536;
+	const unsigned ck = 64;
+	const size_t slot = slot_size(S, ck);
+	const unsigned R_max = kTuneR[kTuneBuckets - 1];
+	uint8_t *in_log = nullptr, *in_dat = nullptr, *outb = nullptr;
+	uint16_t *logs = nullptr, *exps = nullptr;
+	hipStream_t s = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	auto cleanup = [&]() {
+		if(in_log) (void)hipFree(in_log);
+		if(in_dat) (void)hipFree(in_dat);
+		if(outb) (void)hipFree(outb);
+		if(logs) (void)hipFree(logs);
+		if(exps) (void)hipFree(exps);
+		if(e0) (void)hipEventDestroy(e0);
+		if(e1) (void)hipEventDestroy(e1);
+		if(s) (void)hipStreamDestroy(s);
+	};
+	if(hipMalloc(&in_log, N * slot) != hipSuccess || hipMalloc(&in_dat, N * slot) != hipSuccess ||
+	   hipMalloc(&outb, (size_t)R_max * slot) != hipSuccess || hipMalloc(&logs, N * 2) != hipSuccess ||
+	   hipMalloc(&exps, R_max * 2) != hipSuccess) {
+		cleanup();
+		return PPH_E_NOMEM;
+	}
+	std::vector<uint16_t> h(std::max(N, R_max));
+	hipError_t e = hipSuccess;
+	for(unsigned i = 0; i < N; i++) h[i] = host_input_log((uint16_t)i);
+	if(e == hipSuccess) e = hipMemcpy(logs, h.data(), N * 2, hipMemcpyHostToDevice);
+	for(unsigned r = 0; r < R_max; r++) h[r] = (uint16_t)r;
+	if(e == hipSuccess) e = hipMemcpy(exps, h.data(), R_max * 2, hipMemcpyHostToDevice);
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipEventCreate(&e0);
+	if(e == hipSuccess) e = hipEventCreate(&e1);
+	if(e == hipSuccess) e = launch_fill_random(in_dat, slot, N, S, 12345, 0, 0, s);
+	if(e == hipSuccess) e = launch_prep(in_log, in_dat, slot, N, nullptr, nullptr, S, ck_offset(S), ck, true, *t, s);
+	if(e == hipSuccess) e = launch_prep(in_dat, nullptr, slot, N, nullptr, nullptr, S, ck_offset(S), ck, false, *t, s);
+	auto timed = [&](int form, unsigned R, float* ms) -> hipError_t {
+		hipError_t r = hipSuccess;
+		for(int rep = 0; rep < 2 && r == hipSuccess; rep++) {
+			r = hipEventRecord(e0, s);
+			if(r != hipSuccess) break;
+			if(form == 0)
+				r = launch_muladd_plain((const uint16_t*)in_log, slot / 2, N, logs, exps, nullptr, 0, (uint16_t*)outb, R,
+				                        false, 8, *t, s);
+			else if(form == 1)
+				r = launch_muladd_plain((const uint16_t*)in_log, slot / 2, N, logs, exps, nullptr, 0, (uint16_t*)outb, R,
+				                        false, 16, *t, s);
+			else
+				r = launch_muladd_dense((const uint16_t*)in_dat, slot / 2, N, logs, exps, (uint16_t*)outb, R, false, *t,
+				                        s);
+			if(r == hipSuccess) r = hipEventRecord(e1, s);
+			if(r == hipSuccess) r = hipEventSynchronize(e1);
+			if(r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
+		}
+		return r;
+	};
+	pph_tuning tu;
+	memset(&tu, 0, sizeof(tu));
+	tu.version = 1;
+	tu.r_buckets = kTuneBuckets;
+	for(unsigned b = 0; b < kTuneBuckets && e == hipSuccess; b++) {
+		const unsigned R = kTuneR[b];
+		const double ma = (double)N * R * (S / 2);
+		float ms = 0;
+		e = timed(0, R, &ms);
+		tu.plain_g8[b] = (float)(ma / (ms * 1e-3));
+		if(e == hipSuccess) e = timed(1, R, &ms);
+		tu.plain_g16[b] = (float)(ma / (ms * 1e-3));
+		if(e == hipSuccess) e = timed(2, R, &ms);
+		tu.dense[b] = (float)(ma / (ms * 1e-3));
+	}
+	cleanup();
+	if(e != hipSuccess) return set_error(e);
+	*out = tu;
+	return PPH_OK;
+}
+
+}  // namespace pph
+
+using namespace pph;
+
+static std::mutex g_tune_mu;
+static bool g_tune_valid = false;
+static pph_tuning g_tune;
+
+extern "C" int pph_autotune(pph_tuning* out) {
+	if(!out) return PPH_E_INVALID;
+	int n = 0;
+	if(hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PPH_E_UNSUPPORTED;
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	int rc = autotune(out);
+	if(rc == PPH_OK) {
+		g_tune = *out;
+		g_tune_valid = true;
+	}
+	return rc;
+}
+
+// ====================================================================== batch
+
+struct pph_batch {
+	int device = 0;
+	unsigned ck = 64;
+	std::vector<pph_job> jobs;
+	std::vector<int> form;         // PPH_FORM_PLAIN / PPH_FORM_DENSE per job
+	unsigned group = 16;           // plain kernel output grouping
+	DeviceTables* tab = nullptr;
+	BatchJob* d_jobs = nullptr;
+	BatchItem* d_items = nullptr;
+	unsigned n_items = 0;
+	unsigned* d_counter = nullptr;
+	hipStream_t cap = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	double predicted_s = 0;
+	hipEvent_t ev_a = nullptr, ev_b = nullptr;  // around the fused kernel (eager runs)
+	bool timing = false;
+};
+
+static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
+	hipError_t e = hipSuccess;
+	if(b->n_items) {
+		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, b->d_counter, 0u);
+		e = hipGetLastError();
+	}
+	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(!jb.n_in || !jb.n_out) continue;
+		e = launch_prep((uint8_t*)jb.in_slots, nullptr, jb.slot_bytes, jb.n_in, jb.in_lens, nullptr, jb.slice_size,
+		                ck_offset(jb.slice_size), b->ck, b->form[j] == PPH_FORM_PLAIN, *b->tab, s);
+	}
+	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_a, s);
+	if(e == hipSuccess && b->n_items) {
+		dim3 grid(b->tab->cus > 0 ? b->tab->cus : 256), block(kPlainThreads);
+		if(b->group == 8)
+			hipLaunchKernelGGL(k_muladd_batch<8>, grid, block, 0, s, b->d_jobs, b->d_items, b->n_items, b->d_counter,
+			                   b->tab->exp);
+		else
+			hipLaunchKernelGGL(k_muladd_batch<16>, grid, block, 0, s, b->d_jobs, b->d_items, b->n_items, b->d_counter,
+			                   b->tab->exp);
+		e = hipGetLastError();
+	}
+	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(b->form[j] != PPH_FORM_DENSE || !jb.n_out) continue;
+		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
+		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, s);
+	}
+	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_b, s);
+	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(!jb.n_out) continue;
+		if(!jb.n_in) continue;
+		e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size, ck_offset(jb.slice_size),
+		                  b->ck, jb.valid, *b->tab, s);
+	}
+	return e;
+}
+
+extern "C" void pph_batch_destroy(pph_batch* b) {
+	if(!b) return;
+	(void)hipSetDevice(b->device);
+	if(b->cap) (void)hipStreamSynchronize(b->cap);
+	if(b->exec) (void)hipGraphExecDestroy(b->exec);
+	if(b->graph) (void)hipGraphDestroy(b->graph);
+	if(b->cap) (void)hipStreamDestroy(b->cap);
+	if(b->d_jobs) (void)hipFree(b->d_jobs);
+	if(b->d_items) (void)hipFree(b->d_items);
+	if(b->d_counter) (void)hipFree(b->d_counter);
+	if(b->ev_a) (void)hipEventDestroy(b->ev_a);
+	if(b->ev_b) (void)hipEventDestroy(b->ev_b);
+	delete b;
+}
+
+extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned cksum_size, const pph_tuning* tuning,
+                                pph_batch** out) {
+	if(!out || (!jobs && n_jobs) || !valid_ck(cksum_size)) return PPH_E_INVALID;
+	*out = nullptr;
+	for(unsigned j = 0; j < n_jobs; j++) {
+		const pph_job& jb = jobs[j];
+		if(jb.n_in > 32768 || jb.n_out > 65535) return PPH_E_INVALID;
+		if(jb.slot_bytes % kSlotAlign || jb.slot_bytes < slot_size(jb.slice_size, cksum_size)) return PPH_E_INVALID;
+		if(jb.n_out && (!jb.out_slots || !jb.valid)) return PPH_E_INVALID;
+		if(jb.n_out && (!jb.n_in || !jb.in_slots || !jb.in_logs || !jb.exps)) return PPH_E_INVALID;
+	}
+	int dev = 0;
+	if(hipGetDevice(&dev) != hipSuccess) return PPH_E_UNSUPPORTED;
+	pph_tuning tu;
+	if(tuning) {
+		tu = *tuning;
+	} else {
+		std::lock_guard<std::mutex> lk(g_tune_mu);
+		if(!g_tune_valid) {
+			int rc = autotune(&g_tune);
+			if(rc) return rc;
+			g_tune_valid = true;
+		}
+		tu = g_tune;
+	}
+	auto* b = new(std::nothrow) pph_batch;
+	if(!b) return PPH_E_NOMEM;
+	b->device = dev;
+	b->ck = cksum_size;
+	b->jobs.assign(jobs, jobs + n_jobs);
+	int rc = device_tables(&b->tab);
+	if(rc) {
+		pph_batch_destroy(b);
+		return rc;
+	}
+
+	// choose the plain grouping for the batch, then the form per job, from the
+	// measured rates (mul-adds per second by recovery-count bucket)
+	auto ma_of = [](const pph_job& jb) { return (double)jb.n_in * jb.n_out * (double)(jb.slot_bytes / 2); };
+	double t8 = 0, t16 = 0;
+	for(const auto& jb : b->jobs) {
+		if(!jb.n_in || !jb.n_out) continue;
+		const unsigned bk = tune_bucket(jb.n_out);
+		t8 += ma_of(jb) / std::max(1.0f, tu.plain_g8[bk]);
+		t16 += ma_of(jb) / std::max(1.0f, tu.plain_g16[bk]);
+	}
+	b->group = t8 < t16 ? 8 : 16;
+	std::vector<double> pred(n_jobs, 0.0);
+	b->form.assign(n_jobs, PPH_FORM_PLAIN);
+	for(unsigned j = 0; j < n_jobs; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(!jb.n_in || !jb.n_out) continue;
+		const unsigned bk = tune_bucket(jb.n_out);
+		const double tp = ma_of(jb) / std::max(1.0f, b->group == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
+		const double td = ma_of(jb) / std::max(1.0f, tu.dense[bk]);
+		b->form[j] = td < tp ? PPH_FORM_DENSE : PPH_FORM_PLAIN;
+		pred[j] = std::min(tp, td);
+		b->predicted_s += pred[j];
+	}
+
+	// work items of the plain jobs, longest job first (largest first keeps the
+	// tail of the persistent launch short); within a job, output groups of one
+	// column tile are adjacent so their shared input reads meet in L2
+	std::vector<unsigned> order;
+	for(unsigned j = 0; j < n_jobs; j++)
+		if(b->form[j] == PPH_FORM_PLAIN && b->jobs[j].n_in && b->jobs[j].n_out) order.push_back(j);
+	std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned c) { return pred[a] > pred[c]; });
+	std::vector<BatchItem> items;
+	std::vector<BatchJob> hj(n_jobs);
+	const unsigned tile_words = kPlainThreads * kPlainWordsPerLane;
+	for(unsigned j = 0; j < n_jobs; j++) {
+		const pph_job& jb = b->jobs[j];
+		hj[j] = BatchJob{(const uint16_t*)jb.in_slots, (uint16_t*)jb.out_slots, jb.in_logs, jb.exps,
+		                 jb.slot_bytes / 2, jb.n_in, jb.n_out};
+	}
+	for(unsigned j : order) {
+		const pph_job& jb = b->jobs[j];
+		const unsigned cols = (unsigned)((jb.slot_bytes / 2 + tile_words - 1) / tile_words);
+		const unsigned grps = (jb.n_out + b->group - 1) / b->group;
+		for(unsigned c = 0; c < cols; c++)
+			for(unsigned gp = 0; gp < grps; gp++) items.push_back(BatchItem{j, gp, c, 0});
+	}
+	b->n_items = (unsigned)items.size();
+	hipError_t e = hipSuccess;
+	if(n_jobs) e = hipMalloc(&b->d_jobs, n_jobs * sizeof(BatchJob));
+	if(e == hipSuccess && b->n_items) e = hipMalloc(&b->d_items, items.size() * sizeof(BatchItem));
+	if(e == hipSuccess) e = hipMalloc(&b->d_counter, 256);
+	if(e == hipSuccess && n_jobs) e = hipMemcpy(b->d_jobs, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice);
+	if(e == hipSuccess && b->n_items)
+		e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipEventCreate(&b->ev_a);
+	if(e == hipSuccess) e = hipEventCreate(&b->ev_b);
+	if(e != hipSuccess) {
+		pph_batch_destroy(b);
+		return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
+	}
+	*out = b;
+	return PPH_OK;
+}
+
+extern "C" int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group, double* predicted_s) {
+	if(!b || job >= b->jobs.size()) return PPH_E_INVALID;
+	if(form) *form = b->form[job];
+	if(group) *group = b->group;
+	if(predicted_s) *predicted_s = b->predicted_s;
+	return PPH_OK;
+}
+
+extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
+	if(!b) return PPH_E_INVALID;
+	hipStream_t s = (hipStream_t)stream;
+	if(!use_graph) {
+		b->timing = true;
+		hipError_t e = enqueue_batch(b, s);
+		b->timing = false;
+		if(e != hipSuccess) return set_error(e);
+		return PPH_OK;
+	}
+	if(!b->exec) {
+		hipError_t e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+		if(e != hipSuccess) return set_error(e);
+		hipError_t ee = enqueue_batch(b, b->cap);
+		e = hipStreamEndCapture(b->cap, &b->graph);
+		if(ee != hipSuccess) return set_error(ee);
+		if(e != hipSuccess) return set_error(e);
+		e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+		if(e != hipSuccess) return set_error(e);
+	}
+	hipError_t e = hipGraphLaunch(b->exec, s);
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+extern "C" int pph_batch_fused_ms(pph_batch* b, float* ms) {
+	if(!b || !ms) return PPH_E_INVALID;
+	hipError_t e = hipEventSynchronize(b->ev_b);
+	if(e == hipSuccess) e = hipEventElapsedTime(ms, b->ev_a, b->ev_b);
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}

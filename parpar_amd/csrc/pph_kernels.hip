@@ -2,6 +2,7 @@
 // gf16_cksum, log-domain conversion), the plain mul-add and checksum
 // verification.  Launchers validate shapes on the host before launching.
 #include "pph_internal.h"
+#include "pph_plain_tile.h"
 
 namespace pph {
 
@@ -304,26 +305,6 @@ hipError_t launch_verify(const uint8_t* slots, size_t slot_bytes, unsigned count
 
 // ------------------------------------------------------------------ plain mul-add
 
-// out[r] (^)= sum_i c(r,i) * in[i] in GF(2^16).  Inputs are in log domain.
-// Each lane owns 8 consecutive words of a 8192-word column tile and G recovery
-// slices; the full 64 Ki-entry exp table sits in LDS (entry 65535 = 0), so one
-// product is   exp[min(lx + lc, lx + lc - 65535, 65535)]   with doubled
-// (byte-address) logs: two adds and a min3, no branch for zero words.
-// Coefficients are uniform across the wavefront: either computed from the
-// input log and the recovery exponent on the scalar unit (PAR2 coefficients,
-// gf16/gfmat_coeff.c:60-77) or read from an explicit table.
-__device__ __forceinline__ uint32_t mod65535(uint32_t x) {
-	x = (x & 0xFFFFu) + (x >> 16);
-	x = (x & 0xFFFFu) + (x >> 16);
-	return x;  // <= 65535, congruent; 65535 acts as 0 in the lookup below
-}
-
-__device__ __forceinline__ uint32_t exp_pair(const uint16_t* s_exp, uint32_t a, uint32_t b) {
-	uint32_t lo = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + a);
-	uint32_t hi = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + b);
-	return lo | (hi << 16);
-}
-
 template <int G, bool TABLE>
 __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
     const uint16_t* __restrict__ in, size_t slot_words, unsigned n_in, const uint16_t* __restrict__ in_logs,
@@ -331,11 +312,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
     uint16_t* __restrict__ out, unsigned n_out, int accumulate, const uint16_t* __restrict__ exp_tab,
     unsigned col_tiles, unsigned groups) {
 	__shared__ uint16_t s_exp[65536];
-	{
-		const uint4* src = reinterpret_cast<const uint4*>(exp_tab);
-		uint4* dst = reinterpret_cast<uint4*>(s_exp);
-		for(unsigned k = threadIdx.x; k < 65536 * 2 / 16; k += kPlainThreads) dst[k] = src[k];
-	}
+	load_exp_table(s_exp, exp_tab);
 	__syncthreads();
 
 	// blocks of one column tile run on the same XCD (blockIdx round-robins over 8 XCDs)
@@ -344,71 +321,8 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
 	if((total & 7u) == 0) L = (blockIdx.x & 7u) * (total >> 3) + (blockIdx.x >> 3);
 	const unsigned col = L / groups;
 	const unsigned grp = L - col * groups;
-	const size_t w0 = (size_t)col * (kPlainThreads * kPlainWordsPerLane) + (size_t)threadIdx.x * kPlainWordsPerLane;
-	const bool active = w0 < slot_words;
-	const unsigned r0 = grp * G;
-
-	uint32_t e[G];
-	if constexpr(!TABLE) {
-#pragma unroll
-		for(int r = 0; r < G; r++) e[r] = (r0 + r < n_out) ? exps[r0 + r] : 0u;
-	}
-
-	uint32_t acc[G][4];
-#pragma unroll
-	for(int r = 0; r < G; r++) {
-		acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
-		if(accumulate && active && r0 + r < n_out) {
-			uint4 v = *reinterpret_cast<const uint4*>(out + (size_t)(r0 + r) * slot_words + w0);
-			acc[r][0] = v.x; acc[r][1] = v.y; acc[r][2] = v.z; acc[r][3] = v.w;
-		}
-	}
-
-	for(unsigned i = 0; i < n_in; i++) {
-		uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-		if(active) v = *reinterpret_cast<const uint4*>(in + (size_t)i * slot_words + w0);
-		const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
-		uint32_t lx[8], lxm[8];
-#pragma unroll
-		for(int j = 0; j < 4; j++) {
-			const uint32_t L0 = wv[j] & 0xFFFFu, L1 = wv[j] >> 16;
-			lx[2 * j] = L0 == kLogZero ? kLogZeroDecoded : 2 * L0;
-			lx[2 * j + 1] = L1 == kLogZero ? kLogZeroDecoded : 2 * L1;
-			lxm[2 * j] = lx[2 * j] - kZeroIdx2;
-			lxm[2 * j + 1] = lx[2 * j + 1] - kZeroIdx2;
-		}
-		uint32_t lin = 0;
-		if constexpr(!TABLE) lin = in_logs[i];
-#pragma unroll
-		for(int r = 0; r < G; r++) {
-			uint32_t lc;
-			bool use;
-			if constexpr(TABLE) {
-				lc = coef[(size_t)i * coef_stride + r0 + r];
-				use = lc != kSkipCoef;
-			} else {
-				lc = 2 * mod65535(lin * e[r]);
-				use = r0 + r < n_out;
-			}
-			if(use) {
-#pragma unroll
-				for(int j = 0; j < 4; j++) {
-					const uint32_t a = min(min(lx[2 * j] + lc, lxm[2 * j] + lc), kZeroIdx2);
-					const uint32_t b = min(min(lx[2 * j + 1] + lc, lxm[2 * j + 1] + lc), kZeroIdx2);
-					acc[r][j] ^= exp_pair(s_exp, a, b);
-				}
-			}
-		}
-	}
-
-	if(active) {
-#pragma unroll
-		for(int r = 0; r < G; r++) {
-			if(r0 + r < n_out)
-				*reinterpret_cast<uint4*>(out + (size_t)(r0 + r) * slot_words + w0) =
-				    make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
-		}
-	}
+	plain_tile<G, TABLE>(s_exp, in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, col,
+	                     grp);
 }
 
 template <int G>

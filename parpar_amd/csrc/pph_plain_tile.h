@@ -1,0 +1,109 @@
+// The plain-form mul-add tile, shared by the standalone kernel and the
+// persistent batched kernel.
+#pragma once
+#include "pph_internal.h"
+
+namespace pph {
+
+// out[r] (^)= sum_i c(r,i) * in[i] in GF(2^16).  Inputs are in log domain.
+// Each lane owns 8 consecutive words of a 8192-word column tile and G recovery
+// slices; the full 64 Ki-entry exp table sits in LDS (entry 65535 = 0), so one
+// product is   exp[min(lx + lc, lx + lc - 65535, 65535)]   with doubled
+// (byte-address) logs: two adds and a min3, no branch for zero words.
+// Coefficients are uniform across the wavefront: either computed from the
+// input log and the recovery exponent on the scalar unit (PAR2 coefficients,
+// gf16/gfmat_coeff.c:60-77) or read from an explicit table.
+__device__ __forceinline__ uint32_t mod65535(uint32_t x) {
+	x = (x & 0xFFFFu) + (x >> 16);
+	x = (x & 0xFFFFu) + (x >> 16);
+	return x;  // <= 65535, congruent; 65535 acts as 0 in the lookup below
+}
+
+__device__ __forceinline__ uint32_t exp_pair(const uint16_t* s_exp, uint32_t a, uint32_t b) {
+	uint32_t lo = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + a);
+	uint32_t hi = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + b);
+	return lo | (hi << 16);
+}
+
+// Load the exp table into LDS (kPlainThreads threads).
+__device__ __forceinline__ void load_exp_table(uint16_t* s_exp, const uint16_t* __restrict__ exp_tab) {
+	const uint4* src = reinterpret_cast<const uint4*>(exp_tab);
+	uint4* dst = reinterpret_cast<uint4*>(s_exp);
+	for(unsigned k = threadIdx.x; k < 65536 * 2 / 16; k += kPlainThreads) dst[k] = src[k];
+}
+
+// One work item: column tile `col` (8192 words) x recovery slices [G*grp, G*grp+G).
+template <int G, bool TABLE>
+__device__ __forceinline__ void plain_tile(const uint16_t* s_exp, const uint16_t* __restrict__ in, size_t slot_words,
+                                           unsigned n_in, const uint16_t* __restrict__ in_logs,
+                                           const uint16_t* __restrict__ exps, const uint32_t* __restrict__ coef,
+                                           unsigned coef_stride, uint16_t* __restrict__ out, unsigned n_out,
+                                           int accumulate, unsigned col, unsigned grp) {
+	const size_t w0 = (size_t)col * (kPlainThreads * kPlainWordsPerLane) + (size_t)threadIdx.x * kPlainWordsPerLane;
+	const bool active = w0 < slot_words;
+	const unsigned r0 = grp * G;
+
+	uint32_t e[G];
+	if constexpr(!TABLE) {
+#pragma unroll
+		for(int r = 0; r < G; r++) e[r] = (r0 + r < n_out) ? exps[r0 + r] : 0u;
+	}
+
+	uint32_t acc[G][4];
+#pragma unroll
+	for(int r = 0; r < G; r++) {
+		acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
+		if(accumulate && active && r0 + r < n_out) {
+			uint4 v = *reinterpret_cast<const uint4*>(out + (size_t)(r0 + r) * slot_words + w0);
+			acc[r][0] = v.x; acc[r][1] = v.y; acc[r][2] = v.z; acc[r][3] = v.w;
+		}
+	}
+
+	for(unsigned i = 0; i < n_in; i++) {
+		uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+		if(active) v = *reinterpret_cast<const uint4*>(in + (size_t)i * slot_words + w0);
+		const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+		uint32_t lx[8], lxm[8];
+#pragma unroll
+		for(int j = 0; j < 4; j++) {
+			const uint32_t L0 = wv[j] & 0xFFFFu, L1 = wv[j] >> 16;
+			lx[2 * j] = L0 == kLogZero ? kLogZeroDecoded : 2 * L0;
+			lx[2 * j + 1] = L1 == kLogZero ? kLogZeroDecoded : 2 * L1;
+			lxm[2 * j] = lx[2 * j] - kZeroIdx2;
+			lxm[2 * j + 1] = lx[2 * j + 1] - kZeroIdx2;
+		}
+		uint32_t lin = 0;
+		if constexpr(!TABLE) lin = in_logs[i];
+#pragma unroll
+		for(int r = 0; r < G; r++) {
+			uint32_t lc;
+			bool use;
+			if constexpr(TABLE) {
+				lc = coef[(size_t)i * coef_stride + r0 + r];
+				use = lc != kSkipCoef;
+			} else {
+				lc = 2 * mod65535(lin * e[r]);
+				use = r0 + r < n_out;
+			}
+			if(use) {
+#pragma unroll
+				for(int j = 0; j < 4; j++) {
+					const uint32_t a = min(min(lx[2 * j] + lc, lxm[2 * j] + lc), kZeroIdx2);
+					const uint32_t b = min(min(lx[2 * j + 1] + lc, lxm[2 * j + 1] + lc), kZeroIdx2);
+					acc[r][j] ^= exp_pair(s_exp, a, b);
+				}
+			}
+		}
+	}
+
+	if(active) {
+#pragma unroll
+		for(int r = 0; r < G; r++) {
+			if(r0 + r < n_out)
+				*reinterpret_cast<uint4*>(out + (size_t)(r0 + r) * slot_words + w0) =
+				    make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+		}
+	}
+}
+
+}  // namespace pph
