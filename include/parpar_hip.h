@@ -178,6 +178,17 @@ int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in,
 int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 
+/* Per-slice hashes of the input path (hasher/hasher_base.h:83-198): for slot k,
+ * MD5 (md5_out + 16k) and CRC-32 (crc_out[k]) of bytes [0, lens[k]) zero-padded
+ * to slice_size (lens: device array, NULL = slice_size; slice_size % 4 == 0).
+ * One lane per slice.  Run on data-form slots, before log-domain preparation. */
+int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens, size_t slice_size,
+                   uint8_t* md5_out, uint32_t* crc_out, void* stream);
+/* MD5 of a file laid out over `count` slots, slice_size bytes each (the last
+ * one partial): file_size bytes in total.  One serial chain on the scalar unit. */
+int pph_file_md5(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size, uint64_t file_size,
+                 uint8_t* md5_out, void* stream);
+
 /* gf16_cksum verification of n slots (copy_cksum_check, gf16/gf16_cksum_base.h:148-159):
  * valid[k] (device int array) = 1 if the stored block matches the slice data. */
 int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,

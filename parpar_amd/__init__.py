@@ -99,6 +99,8 @@ _SIGS = {
     "pph_batch_run": (_i, [_vp, _i, _vp]),
     "pph_batch_destroy": (None, [_vp]),
     "pph_batch_fused_ms": (_i, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "pph_slice_hash": (_i, [_vp, _sz, _u, _vp, _sz, _vp, _vp, _vp]),
+    "pph_file_md5": (_i, [_vp, _sz, _u, _sz, _u64, _vp, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
 }
 

@@ -32,7 +32,16 @@ struct HostTables {
 	uint16_t log[65536];
 	uint16_t in_log[32768];
 	uint16_t rowmask[16 * 64];
+	uint32_t crc[4 * 256];
 	HostTables() {
+		for(uint32_t i = 0; i < 256; i++) {
+			uint32_t c = i;
+			for(int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : (c >> 1);
+			crc[i] = c;
+		}
+		for(int tab = 1; tab < 4; tab++)
+			for(uint32_t i = 0; i < 256; i++)
+				crc[tab * 256 + i] = (crc[(tab - 1) * 256 + i] >> 8) ^ crc[crc[(tab - 1) * 256 + i] & 255];
 		uint32_t n = 1;
 		for(uint32_t e = 0; e < kOrder; e++) {
 			exp[e] = (uint16_t)n;
@@ -81,6 +90,7 @@ static const HostTables& tables() {
 const uint16_t* host_exp() { return tables().exp; }
 const uint16_t* host_log() { return tables().log; }
 const uint16_t* host_rowmask() { return tables().rowmask; }
+const uint32_t* host_crc() { return tables().crc; }
 uint16_t host_input_log(uint16_t input) { return tables().in_log[input & 32767]; }
 uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent) {
 	uint32_t r = (uint32_t)input_log * exponent;
@@ -103,12 +113,17 @@ int device_tables(DeviceTables** out) {
 	if(!t.exp) {
 		const HostTables& h = tables();
 		uint16_t *e = nullptr, *l = nullptr, *rm = nullptr;
+		uint32_t* cr = nullptr;
 		PPH_HIP(hipMalloc(&e, sizeof(h.exp)));
-		if(hipMalloc(&l, sizeof(h.log)) != hipSuccess || hipMalloc(&rm, sizeof(h.rowmask)) != hipSuccess) {
+		if(hipMalloc(&l, sizeof(h.log)) != hipSuccess || hipMalloc(&rm, sizeof(h.rowmask)) != hipSuccess ||
+		   hipMalloc(&cr, sizeof(h.crc)) != hipSuccess) {
 			(void)hipFree(e);
 			if(l) (void)hipFree(l);
+			if(rm) (void)hipFree(rm);
 			return PPH_E_NOMEM;
 		}
+		PPH_HIP(hipMemcpy(cr, h.crc, sizeof(h.crc), hipMemcpyHostToDevice));
+		t.crc = cr;
 		PPH_HIP(hipMemcpy(e, h.exp, sizeof(h.exp), hipMemcpyHostToDevice));
 		PPH_HIP(hipMemcpy(l, h.log, sizeof(h.log), hipMemcpyHostToDevice));
 		PPH_HIP(hipMemcpy(rm, h.rowmask, sizeof(h.rowmask), hipMemcpyHostToDevice));
@@ -324,6 +339,29 @@ extern "C" int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigne
 	if(rc) return rc;
 	hipError_t e = launch_muladd_dense((const uint16_t*)in_slots, slot_bytes / 2, n_in, input_logs, exps,
 	                                   (uint16_t*)out_slots, n_out, accumulate != 0, *t, (hipStream_t)stream);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+extern "C" int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                              size_t slice_size, uint8_t* md5_out, uint32_t* crc_out, void* stream) {
+	if((!slots && count) || (count && (!md5_out || !crc_out))) return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	hipError_t e = launch_slice_hash((const uint8_t*)slots, slot_bytes, count, lens, slice_size, md5_out, crc_out, *t,
+	                                 (hipStream_t)stream);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+extern "C" int pph_file_md5(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
+                            uint64_t file_size, uint8_t* md5_out, void* stream) {
+	if(!slots || !md5_out) return PPH_E_INVALID;
+	hipError_t e = launch_file_md5((const uint8_t*)slots, slot_bytes, count, slice_size, file_size, md5_out,
+	                               (hipStream_t)stream);
 	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
 	if(e != hipSuccess) return set_error(e);
 	return PPH_OK;

@@ -96,7 +96,7 @@ int autotune(pph_tuning* out) {
 		if(s) (void)hipStreamDestroy(s);
 	};
 	if(hipMalloc(&in_log, N * slot) != hipSuccess || hipMalloc(&in_dat, N * slot) != hipSuccess ||
-	   hipMalloc(&outb, (size_t)R_max * slot) != hipSuccess || hipMalloc(&logs, N * 2) != hipSuccess ||
+	   hipMalloc(&outb, (size_t)1 << 30) != hipSuccess || hipMalloc(&logs, N * 2) != hipSuccess ||
 	   hipMalloc(&exps, R_max * 2) != hipSuccess) {
 		cleanup();
 		return PPH_E_NOMEM;
@@ -113,41 +113,86 @@ int autotune(pph_tuning* out) {
 	if(e == hipSuccess) e = launch_fill_random(in_dat, slot, N, S, 12345, 0, 0, s);
 	if(e == hipSuccess) e = launch_prep(in_log, in_dat, slot, N, nullptr, nullptr, S, ck_offset(S), ck, true, *t, s);
 	if(e == hipSuccess) e = launch_prep(in_dat, nullptr, slot, N, nullptr, nullptr, S, ck_offset(S), ck, false, *t, s);
-	auto timed = [&](int form, unsigned R, float* ms) -> hipError_t {
+	// The plain form runs as the persistent batched kernel: measure it that way,
+	// on J jobs that share the input slots, enough work items to fill the GPU.
+	BatchJob* d_jobs = nullptr;
+	BatchItem* d_items = nullptr;
+	unsigned* d_counter = nullptr;
+	const size_t out_cap = (size_t)1 << 30;
+	auto timed = [&](int form, unsigned R, float* ms, double* ma) -> hipError_t {
 		hipError_t r = hipSuccess;
+		const unsigned G = form == 0 ? 8 : 16;
+		const unsigned tile_words = kPlainThreads * kPlainWordsPerLane;
+		const unsigned cols = (unsigned)((slot / 2 + tile_words - 1) / tile_words);
+		const unsigned per_job = cols * ((R + G - 1) / G);
+		unsigned J = (2048 + per_job - 1) / per_job;
+		const unsigned jmax = (unsigned)std::max<size_t>(1, out_cap / ((size_t)R * slot));
+		if(form == 2) J = 1;
+		J = std::min(J, jmax);
+		*ma = (double)J * N * R * (S / 2);
+		if(form != 2) {
+			std::vector<BatchJob> hj(J);
+			std::vector<BatchItem> hi;
+			for(unsigned j = 0; j < J; j++) {
+				hj[j] = BatchJob{(const uint16_t*)in_log, (uint16_t*)(outb + (size_t)j * R * slot), logs, exps, slot / 2, N, R};
+				for(unsigned c = 0; c < cols; c++)
+					for(unsigned gp = 0; gp < (R + G - 1) / G; gp++) hi.push_back(BatchItem{j, gp, c, 0});
+			}
+			if(d_jobs) (void)hipFree(d_jobs);
+			if(d_items) (void)hipFree(d_items);
+			d_jobs = nullptr;
+			d_items = nullptr;
+			r = hipMalloc(&d_jobs, J * sizeof(BatchJob));
+			if(r == hipSuccess) r = hipMalloc(&d_items, hi.size() * sizeof(BatchItem));
+			if(r == hipSuccess) r = hipMemcpy(d_jobs, hj.data(), J * sizeof(BatchJob), hipMemcpyHostToDevice);
+			if(r == hipSuccess) r = hipMemcpy(d_items, hi.data(), hi.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+			for(int rep = 0; rep < 2 && r == hipSuccess; rep++) {
+				hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, d_counter, 0u);
+				r = hipEventRecord(e0, s);
+				if(r != hipSuccess) break;
+				dim3 grid(t->cus > 0 ? t->cus : 256), block(kPlainThreads);
+				if(G == 8)
+					hipLaunchKernelGGL(k_muladd_batch<8>, grid, block, 0, s, d_jobs, d_items, (unsigned)hi.size(), d_counter,
+					                   t->exp);
+				else
+					hipLaunchKernelGGL(k_muladd_batch<16>, grid, block, 0, s, d_jobs, d_items, (unsigned)hi.size(),
+					                   d_counter, t->exp);
+				r = hipGetLastError();
+				if(r == hipSuccess) r = hipEventRecord(e1, s);
+				if(r == hipSuccess) r = hipEventSynchronize(e1);
+				if(r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
+			}
+			return r;
+		}
 		for(int rep = 0; rep < 2 && r == hipSuccess; rep++) {
 			r = hipEventRecord(e0, s);
 			if(r != hipSuccess) break;
-			if(form == 0)
-				r = launch_muladd_plain((const uint16_t*)in_log, slot / 2, N, logs, exps, nullptr, 0, (uint16_t*)outb, R,
-				                        false, 8, *t, s);
-			else if(form == 1)
-				r = launch_muladd_plain((const uint16_t*)in_log, slot / 2, N, logs, exps, nullptr, 0, (uint16_t*)outb, R,
-				                        false, 16, *t, s);
-			else
-				r = launch_muladd_dense((const uint16_t*)in_dat, slot / 2, N, logs, exps, (uint16_t*)outb, R, false, *t,
-				                        s);
+			r = launch_muladd_dense((const uint16_t*)in_dat, slot / 2, N, logs, exps, (uint16_t*)outb, R, false, *t, s);
 			if(r == hipSuccess) r = hipEventRecord(e1, s);
 			if(r == hipSuccess) r = hipEventSynchronize(e1);
 			if(r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
 		}
 		return r;
 	};
+	if(e == hipSuccess) e = hipMalloc(&d_counter, 256);
 	pph_tuning tu;
 	memset(&tu, 0, sizeof(tu));
 	tu.version = 1;
 	tu.r_buckets = kTuneBuckets;
 	for(unsigned b = 0; b < kTuneBuckets && e == hipSuccess; b++) {
 		const unsigned R = kTuneR[b];
-		const double ma = (double)N * R * (S / 2);
+		double ma = 0;
 		float ms = 0;
-		e = timed(0, R, &ms);
+		e = timed(0, R, &ms, &ma);
 		tu.plain_g8[b] = (float)(ma / (ms * 1e-3));
-		if(e == hipSuccess) e = timed(1, R, &ms);
+		if(e == hipSuccess) e = timed(1, R, &ms, &ma);
 		tu.plain_g16[b] = (float)(ma / (ms * 1e-3));
-		if(e == hipSuccess) e = timed(2, R, &ms);
+		if(e == hipSuccess) e = timed(2, R, &ms, &ma);
 		tu.dense[b] = (float)(ma / (ms * 1e-3));
 	}
+	if(d_jobs) (void)hipFree(d_jobs);
+	if(d_items) (void)hipFree(d_items);
+	if(d_counter) (void)hipFree(d_counter);
 	cleanup();
 	if(e != hipSuccess) return set_error(e);
 	*out = tu;

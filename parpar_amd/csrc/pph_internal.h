@@ -29,6 +29,7 @@ struct DeviceTables {
 	uint16_t* exp = nullptr;   // exp[k] = 2^k for k < 65535; exp[65535] = 0
 	uint16_t* log = nullptr;   // log[x] = log2(x) for x != 0; log[0] = 0xFFFF
 	uint16_t* rowmask = nullptr;  // [j][q][n]: row j of the bit-matrix of n * z^(4q), as a mask
+	uint32_t* crc = nullptr;      // CRC-32 slice-by-4 tables [4][256]
 	int cus = 0;
 };
 
@@ -36,6 +37,7 @@ struct DeviceTables {
 const uint16_t* host_exp();
 const uint16_t* host_log();
 const uint16_t* host_rowmask();  // 16 x 4 x 16 nibble tables of the coefficient bit-matrices
+const uint32_t* host_crc();      // CRC-32 (reflected 0xEDB88320) slice-by-4 tables, 4 x 256
 uint16_t host_input_log(uint16_t input);
 uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent);
 
@@ -60,6 +62,11 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
 hipError_t launch_muladd_dense(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
                                const uint16_t* exps, uint16_t* out, unsigned n_out, bool accumulate,
                                const DeviceTables& t, hipStream_t s);
+hipError_t launch_slice_hash(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                             size_t slice_size, uint8_t* md5_out, uint32_t* crc_out, const DeviceTables& t,
+                             hipStream_t s);
+hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
+                           uint64_t file_size, uint8_t* md5_out, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
