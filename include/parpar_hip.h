@@ -124,6 +124,8 @@ int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t offset, ui
  * Device layer
  * ====================================================================== */
 int pph_set_device(int device);
+/* Free and total device memory of the current device, bytes. */
+int pph_mem_get_info(size_t* free_bytes, size_t* total_bytes);
 int pph_malloc(void** ptr, size_t bytes);
 int pph_free(void* ptr);
 int pph_memcpy_h2d(void* dst, const void* src, size_t bytes);
@@ -185,7 +187,8 @@ int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, con
 int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens, size_t slice_size,
                    uint8_t* md5_out, uint32_t* crc_out, void* stream);
 /* MD5 of a file laid out over `count` slots, slice_size bytes each (the last
- * one partial): file_size bytes in total.  One serial chain on the scalar unit. */
+ * one partial): file_size bytes in total (hasher_base.h:178-186).  One serial
+ * chain, run by one wavefront. */
 int pph_file_md5(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size, uint64_t file_size,
                  uint8_t* md5_out, void* stream);
 
@@ -240,6 +243,75 @@ int pph_batch_run(pph_batch* b, int use_graph, void* stream);
  * events recorded on its stream around them; waits for that run. */
 int pph_batch_fused_ms(pph_batch* b, float* ms);
 void pph_batch_destroy(pph_batch* b);
+
+/* ======================================================================
+ * Serving: jobs arrive over time and run with continuous batching
+ * ====================================================================== */
+/* Engine settings.  hbm_budget bytes of device memory are allocated once at
+ * create and shared by the admitted jobs; a job is admitted when its memory
+ * (2 x inputs + recovery slices + small arrays, in slots) and one of
+ * max_compute compute streams are free. */
+typedef struct pph_serve_config {
+	size_t hbm_budget;
+	unsigned max_compute;    /* 1..64 jobs computing at once */
+	size_t md5_quantum;      /* bytes of each file's MD5 chain per hash round (multiple of 64; 0 = 8 MiB) */
+	size_t staging_bytes;    /* each of the two pinned staging chunks (0 = 64 MiB) */
+} pph_serve_config;
+
+/* One file.  Its slices (slice_size bytes, the last one partial) are PAR2
+ * inputs first_input, first_input + 1, ...; the job computes, for every slice,
+ * MD5 and CRC-32 of the slice zero-padded to slice_size (the IFSC entries,
+ * hasher/hasher_base.h:154-176), the MD5 of the file (hasher_base.h:178-186),
+ * and n_out recovery slices with their gf16_cksum flags (PAR2Proc over the
+ * file's slices, gf16/controller.h:248-326).  Output pointers are host memory,
+ * may be NULL, and are written before the job's result is posted. */
+typedef struct pph_serve_job {
+	uint64_t file_size;         /* >= 1 */
+	size_t slice_size;          /* multiple of 4; ceil(file_size / slice_size) + first_input <= 32768 */
+	unsigned n_out;             /* recovery slices (0: hashes only) */
+	unsigned first_input;
+	const uint16_t* exps;       /* n_out exponents (< 65535), copied at submit; NULL = 0 .. n_out-1 */
+	const void* host_data;      /* file bytes, read until the job completes; NULL = generated on the
+	                               device: slice k = pph_fill_random(seed, job_id, first slice k) */
+	uint64_t seed;
+	uint32_t job_id;
+	void* recovery_out;         /* n_out x slice_size bytes */
+	int* valid;                 /* n_out flags */
+	uint8_t* slice_md5;         /* 16 bytes per slice */
+	uint32_t* slice_crc;        /* one per slice */
+	uint8_t* file_md5;          /* 16 bytes */
+} pph_serve_job;
+
+typedef struct pph_serve_result {
+	uint64_t ticket;
+	int status;                 /* PPH_OK or the error that ended the job */
+	double latency_s;           /* submit -> every output written */
+	double queue_s;             /* submit -> admitted */
+	double compute_s;           /* admitted -> recovery, flags and slice hashes written */
+	double run_s;               /* admitted -> done (includes the file MD5 chain) */
+	double mul_adds;            /* n_in x n_out x slice_size / 2 */
+	double bytes;               /* file_size */
+} pph_serve_result;
+
+typedef struct pph_serve_stats {
+	uint64_t submitted, completed, md5_rounds;
+	unsigned peak_jobs;         /* most jobs admitted at once */
+	uint64_t peak_bytes;        /* most arena bytes in use at once */
+	uint64_t arena_bytes;
+	double mul_adds, bytes;     /* of the jobs completed without error */
+} pph_serve_stats;
+
+typedef struct pph_serve pph_serve;
+int pph_serve_create(int device, const pph_serve_config* cfg, pph_serve** out);
+/* Queue a job (thread-safe).  PPH_E_NOMEM: the job can never fit the budget. */
+int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t* ticket);
+/* Up to max finished jobs' results, waiting up to wait_ms for the first. */
+int pph_serve_poll(pph_serve* s, pph_serve_result* out, unsigned max, unsigned* n, int wait_ms);
+/* Wait up to wait_ms for every submitted job to finish; returns how many have not. */
+int pph_serve_drain(pph_serve* s, int wait_ms);
+int pph_serve_get_stats(pph_serve* s, pph_serve_stats* out);
+/* Finishes every submitted job, then frees the engine. */
+void pph_serve_destroy(pph_serve* s);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,7 @@ _SIGS = {
     "pph_proc_num_recovery_slices": (_i, [_vp]),
     "pph_proc_debug_corrupt_output": (_i, [_vp, _u, _sz, ctypes.c_uint8]),
     "pph_set_device": (_i, [_i]),
+    "pph_mem_get_info": (_i, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
     "pph_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
     "pph_free": (_i, [_vp]),
     "pph_memcpy_h2d": (_i, [_vp, _vp, _sz]),
@@ -102,6 +103,12 @@ _SIGS = {
     "pph_slice_hash": (_i, [_vp, _sz, _u, _vp, _sz, _vp, _vp, _vp]),
     "pph_file_md5": (_i, [_vp, _sz, _u, _sz, _u64, _vp, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
+    "pph_serve_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "pph_serve_submit": (_i, [_vp, _vp, ctypes.POINTER(_u64)]),
+    "pph_serve_poll": (_i, [_vp, _vp, _u, ctypes.POINTER(_u), _i]),
+    "pph_serve_drain": (_i, [_vp, _i]),
+    "pph_serve_get_stats": (_i, [_vp, _vp]),
+    "pph_serve_destroy": (None, [_vp]),
 }
 
 

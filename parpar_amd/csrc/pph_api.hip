@@ -199,6 +199,11 @@ extern "C" int pph_set_device(int device) {
 	PPH_HIP(hipSetDevice(device));
 	return PPH_OK;
 }
+extern "C" int pph_mem_get_info(size_t* free_bytes, size_t* total_bytes) {
+	if(!free_bytes || !total_bytes) return PPH_E_INVALID;
+	PPH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+	return PPH_OK;
+}
 extern "C" int pph_malloc(void** ptr, size_t bytes) {
 	if(!ptr) return PPH_E_INVALID;
 	hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);

@@ -67,8 +67,25 @@ hipError_t launch_slice_hash(const uint8_t* slots, size_t slot_bytes, unsigned c
                              hipStream_t s);
 hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                            uint64_t file_size, uint8_t* md5_out, hipStream_t s);
+// Spans of file MD5 chains (n Md5Span records, device-resident), one wave each.
+hipError_t launch_file_md5_round(const void* spans, unsigned n, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// A span [begin, end) of one file's MD5 chain, the file laid out over slots
+// (slot k holds file bytes [k*slice_size, ...)).  begin is a multiple of 64;
+// end too unless the span is final (then end = file_size and the MD5 padding
+// follows).  The chain state lives in `state` (4 words, device) between
+// spans; a final span writes the digest to `digest` (16 bytes, device).
+struct Md5Span {
+	const uint8_t* slots;
+	uint64_t slot_bytes, slice_size, file_size, begin, end;
+	uint32_t* state;
+	uint8_t* digest;
+	uint32_t flags;  // kMd5SpanFirst | kMd5SpanFinal
+	uint32_t pad;
+};
+constexpr uint32_t kMd5SpanFirst = 1, kMd5SpanFinal = 2;
 
 inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline size_t ck_offset(size_t slice_size) { return round_up(slice_size, 16); }

@@ -1,0 +1,150 @@
+"""Serving engine: jobs submitted over time, admitted by device-memory
+footprint, run concurrently; every output (recovery slices, checksum flags,
+per-slice MD5 / CRC-32, file MD5) checked against the oracle and
+hashlib / zlib, and a job's outputs shown to be the same alone and mixed with
+other jobs.  Calls go through the C ABI (pph_serve_*)."""
+import ctypes
+import hashlib
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import parpar_amd
+from parpar_amd import serve as SV
+from golden_io import expected_valid
+
+pytestmark = pytest.mark.gpu
+MiB = 1 << 20
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def file_bytes(job):
+    if job.data is not None:
+        return job.data
+    S = job.slice_size
+    parts = [O.prng_bytes(job.seed, job.job_id, k, min(S, job.file_size - k * S)) for k in range(job.n_in)]
+    return np.concatenate(parts)
+
+
+def check_job(job, data=None):
+    """Compare a finished job with the oracle and Python's hash routines."""
+    assert job.result is not None and job.result["status"] == 0, job.result
+    data = file_bytes(job) if data is None else data
+    S, n = job.slice_size, job.n_in
+    assert bytes(job.file_md5).hex() == hashlib.md5(data.tobytes()).hexdigest()
+    padded = np.zeros((n, S), np.uint8)
+    padded.reshape(-1)[:job.file_size] = data
+    lens = np.full(n, S, np.uint32)
+    lens[-1] = job.file_size - (n - 1) * S
+    if job.slice_md5 is not None:
+        for k in range(n):
+            row = padded[k].tobytes()
+            assert bytes(job.slice_md5[k]).hex() == hashlib.md5(row).hexdigest(), k
+            assert int(job.slice_crc[k]) == zlib.crc32(row) & 0xffffffff, k
+    if job.n_out:
+        exps = np.arange(job.n_out, dtype=np.uint16) if job.exps is None else job.exps
+        nums = np.arange(job.first_input, job.first_input + n, dtype=np.uint16)
+        if job.recovery is not None:
+            ref = O.recover(padded, lens, S, exps, input_nums=nums)
+            assert (job.recovery == ref).all()
+        flags = expected_valid(padded, lens, S, exps, nums)
+        assert ((job.valid[:job.n_out] == 1) == flags).all()
+
+
+def mixed_jobs():
+    """A spread of shapes: host and device inputs, partial and odd last
+    slices, a single slice, hashes only, explicit exponents, input offsets."""
+    rng = np.random.default_rng(44)
+    jobs = []
+    specs = [
+        # file_size, slice_size, n_out, host?, exps, first_input
+        (5 * 65536 + 1001, 65536, 3, True, None, 0),
+        (40 * 4096, 4096, 7, False, None, 0),
+        (999, 1024, 2, True, [0, 65534], 0),
+        (3, 4, 1, False, None, 0),
+        (17 * 8192 - 7, 8192, 5, False, [1, 2, 3, 100, 200], 100),
+        (100000, 20000, 0, True, None, 0),
+        (64 * 1024 * 3, 64 * 1024, 12, False, list(range(12)), 32760),
+        (2 * MiB + 12, MiB, 2, True, None, 7),
+    ]
+    for j, (fs, S, R, host, exps, first) in enumerate(specs):
+        data = O.prng_bytes(7, 1000 + j, 0, fs) if host else None
+        jobs.append(SV.FileJob(fs, S, R, exps=exps, data=data, seed=5, job_id=j, first_input=first))
+    rng.shuffle(jobs)
+    return jobs
+
+
+def test_serve_mixed_jobs():
+    jobs = mixed_jobs()
+    say("serve mixed", [(j.file_size, j.slice_size, j.n_out) for j in jobs])
+    with SV.Server(hbm_budget=256 * MiB, max_compute=4, md5_quantum=64 * 1024) as srv:
+        srv.run_all(jobs)
+        st = srv.stats()
+    say(" stats", st)
+    assert st["completed"] == len(jobs) and st["peak_bytes"] <= 256 * MiB
+    for job in jobs:
+        check_job(job)
+
+
+def test_serve_alone_vs_mixed():
+    """A job's outputs do not depend on what runs beside it."""
+    def target():
+        return SV.FileJob(9 * 32768 + 123, 32768, 6, seed=11, job_id=77, first_input=3)
+
+    alone = target()
+    with SV.Server(hbm_budget=64 * MiB, max_compute=1) as srv:
+        srv.run_all([alone])
+    check_job(alone)
+
+    mixed = target()
+    others = mixed_jobs()
+    # the target sits among the others; a tight budget forces queueing and bypass
+    order = others[:3] + [mixed] + others[3:]
+    with SV.Server(hbm_budget=24 * MiB, max_compute=3, md5_quantum=4096) as srv:
+        srv.run_all(order)
+        st = srv.stats()
+    say(" mixed stats", st)
+    for name in ("recovery", "valid", "slice_md5", "slice_crc", "file_md5"):
+        assert (getattr(alone, name) == getattr(mixed, name)).all(), name
+    for job in others:
+        check_job(job)
+
+
+@pytest.mark.parametrize("quantum", [64, 4096, 8 * MiB])
+def test_serve_md5_rounds(quantum):
+    """Files of many sizes hash in shared rounds of `quantum` bytes per chain."""
+    sizes = [1, 63, 64, 65, 4095, 4096, 4097, 100003, 3 * 65536]
+    jobs = [SV.FileJob(fs, 65536, 0, data=O.prng_bytes(8, k, 0, fs), job_id=k) for k, fs in enumerate(sizes)]
+    say("md5 rounds", quantum)
+    with SV.Server(hbm_budget=64 * MiB, max_compute=8, md5_quantum=quantum) as srv:
+        srv.run_all(jobs)
+        rounds = srv.stats()["md5_rounds"]
+    say(" rounds", rounds)
+    for job in jobs:
+        assert bytes(job.file_md5).hex() == hashlib.md5(job.data.tobytes()).hexdigest(), job.file_size
+    if quantum == 64:
+        assert rounds >= 3 * 65536 // 64
+
+
+def test_serve_rejects():
+    with SV.Server(hbm_budget=8 * MiB, max_compute=2) as srv:
+        lib = parpar_amd.lib()
+        t = ctypes.c_uint64()
+
+        def submit(job):
+            d = job.desc()
+            return lib.pph_serve_submit(srv.h, ctypes.byref(d), ctypes.byref(t))
+
+        assert submit(SV.FileJob(16 * MiB, MiB, 1)) == parpar_amd.PPH_E_NOMEM  # never fits
+        assert submit(SV.FileJob(1000, 6, 1)) == parpar_amd.PPH_E_INVALID      # slice not a multiple of 4
+        assert submit(SV.FileJob(4 * 32769, 4, 1)) == parpar_amd.PPH_E_INVALID  # more than 32768 slices
+        assert submit(SV.FileJob(400, 4, 1, first_input=32700)) == parpar_amd.PPH_E_INVALID
+        assert submit(SV.FileJob(400, 4, 1, exps=[65535])) == parpar_amd.PPH_E_INVALID
+        ok = SV.FileJob(4000, 1024, 2, seed=1, job_id=1)
+        srv.run_all([ok])
+    check_job(ok)
