@@ -180,6 +180,22 @@ int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in,
 int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 
+/* Bilinear dense form: the same result as pph_muladd_plain / pph_muladd_dense
+ * (data-form input slots, PAR2 coefficients), computed from a rank-81 bilinear
+ * formula for GF(2^16) multiplication (four levels of Karatsuba): 81 batched
+ * int8 GEMMs over 0/1 operands (hipBLASLt) between hand-written expansion and
+ * parity/recombination kernels; 81 MACs per mul-add instead of 256.  Needs a
+ * device workspace (pph_bilinear_workspace bytes), allocated on first use,
+ * kept for later calls and freed by pph_bilinear_release.  Calls on one device
+ * are serialised. */
+int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                        const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
+int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes);
+int pph_bilinear_release(void);
+/* Host reference of the bilinear formula: c * x in GF(2^16) through the
+ * expansion, componentwise product, recombination and reduction. */
+uint16_t pph_bilinear_mul(uint16_t c, uint16_t x);
+
 /* Per-slice hashes of the input path (hasher/hasher_base.h:83-198): for slot k,
  * MD5 (md5_out + 16k) and CRC-32 (crc_out[k]) of bytes [0, lens[k]) zero-padded
  * to slice_size (lens: device array, NULL = slice_size; slice_size % 4 == 0).

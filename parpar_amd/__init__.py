@@ -103,6 +103,10 @@ _SIGS = {
     "pph_slice_hash": (_i, [_vp, _sz, _u, _vp, _sz, _vp, _vp, _vp]),
     "pph_file_md5": (_i, [_vp, _sz, _u, _sz, _u64, _vp, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
+    "pph_muladd_bilinear": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
+    "pph_bilinear_workspace": (_i, [_sz, _u, _u, ctypes.POINTER(_sz)]),
+    "pph_bilinear_release": (_i, []),
+    "pph_bilinear_mul": (_u16, [_u16, _u16]),
     "pph_serve_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
     "pph_serve_submit": (_i, [_vp, _vp, ctypes.POINTER(_u64)]),
     "pph_serve_poll": (_i, [_vp, _vp, _u, ctypes.POINTER(_u), _i]),

@@ -21,7 +21,9 @@ INCLUDE = os.path.join(ROOT, "include")
 
 ARCH = os.environ.get("PPH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIP_SOURCES = ["pph_kernels.hip", "pph_dense.hip", "pph_batch.hip", "pph_hash.hip", "pph_serve.hip", "pph_api.hip"]
+ROCM_LIB = "/opt/rocm/lib"
+HIP_SOURCES = ["pph_kernels.hip", "pph_dense.hip", "pph_batch.hip", "pph_hash.hip", "pph_serve.hip", "pph_bilinear.hip",
+               "pph_api.hip"]
 CXX_SOURCES = ["controller_hip.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              # packed FP32 VALU forms stay off in every device build
@@ -52,7 +54,8 @@ def build_library(verbose=False):
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+         ["-L" + ROCM_LIB, "-lhipblaslt", "-Wl,-rpath," + ROCM_LIB])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -104,3 +104,18 @@ def test_par2prochip_links_with_reference_controller():
                        text=True, timeout=60)
     assert r.returncode == 0, r.stdout
     assert "constructed" in r.stdout
+
+
+def test_bilinear_formula_matches_gf_multiply():
+    """The rank-81 Karatsuba formula behind the bilinear dense form (host copy of
+    its expansion / recombination maps) multiplies like GF(2^16): all 65536
+    coefficients against the basis words and random pairs."""
+    import numpy as np
+    import oracle as O
+    L, R = parpar_amd.lib(), O.lib()
+    for c in range(0, 65536, 7):
+        for b in (0, 3, 8, 15):
+            assert L.pph_bilinear_mul(c, 1 << b) == R.po_mul(c, 1 << b)
+    rng = np.random.default_rng(81)
+    for c, x in rng.integers(0, 65536, (20000, 2)):
+        assert L.pph_bilinear_mul(int(c), int(x)) == R.po_mul(int(c), int(x))

@@ -1,5 +1,6 @@
-"""Dense form parity: the int8-MFMA bit-matrix contraction against the oracle,
-the reference's golden vectors and the plain kernel.  Calls go through the C ABI."""
+"""Dense forms' parity: the int8-MFMA bit-matrix contraction and the bilinear
+(Karatsuba, 81 int8 GEMMs) form against the oracle, the reference's golden
+vectors and the plain kernel.  Calls go through the C ABI."""
 import hashlib
 
 import numpy as np
@@ -22,7 +23,10 @@ def lib():
     return parpar_amd.lib()
 
 
-def run_dense(datas, lens, S, exps, nums=None, ck=64, split=None, also_plain=False):
+FORMS = ["dense", "bilinear"]
+
+
+def run_dense(datas, lens, S, exps, nums=None, ck=64, split=None, also_plain=False, form="dense"):
     n = len(datas)
     _, slot = P.slot_layout(S, ck)
     assert slot % 1024 == 0
@@ -45,8 +49,9 @@ def run_dense(datas, lens, S, exps, nums=None, ck=64, split=None, also_plain=Fal
     out.upload(np.full(slot * n_out, 0x77, np.uint8))
     parts = [(0, n)] if split is None else [(0, split), (split, n)]
     for bi, (a, b) in enumerate(parts):
-        parpar_amd.check(lib().pph_muladd_dense(data.offset(a * slot), slot, b - a, d_logs.offset(2 * a), d_exps.ptr,
-                                                out.ptr, n_out, 1 if bi else 0, None), "dense")
+        fn = lib().pph_muladd_dense if form == "dense" else lib().pph_muladd_bilinear
+        parpar_amd.check(fn(data.offset(a * slot), slot, b - a, d_logs.offset(2 * a), d_exps.ptr,
+                            out.ptr, n_out, 1 if bi else 0, None), form)
     valid = P.DeviceBuffer(4 * n_out)
     parpar_amd.check(lib().pph_verify_slots(out.ptr, slot, n_out, S, ck, valid.ptr, None), "verify")
     parpar_amd.check(lib().pph_synchronize())
@@ -64,20 +69,22 @@ def run_dense(datas, lens, S, exps, nums=None, ck=64, split=None, also_plain=Fal
     return res, v, plain
 
 
-def test_dense_single_input_single_slice():
-    say("dense 1x1")
+@pytest.mark.parametrize("form", FORMS)
+def test_dense_single_input_single_slice(form):
+    say(form, "1x1")
     S = 2048
     d = O.prng_bytes(70, 0, 0, S)
-    got, valid, _ = run_dense([d], [S], S, [1])
+    got, valid, _ = run_dense([d], [S], S, [1], form=form)
     ref = O.recover(d[None, :], [S], S, [1])
     assert (got[0, :S] == ref[0]).all()
     assert valid.all()
 
 
 @pytest.mark.parametrize("n,n_out,S", [(1, 1, 1024), (2, 8, 4096), (3, 9, 4000), (17, 13, 8192), (64, 40, 20000),
-                                       (5, 3, 1)])
-def test_dense_vs_oracle(n, n_out, S):
-    say("dense", n, n_out, S)
+                                       (5, 3, 1), (300, 70, 6000)])
+@pytest.mark.parametrize("form", FORMS)
+def test_dense_vs_oracle(n, n_out, S, form):
+    say(form, n, n_out, S)
     rng = np.random.default_rng(n * 1000 + n_out)
     lens = [S] * n
     lens[-1] = int(rng.integers(0, S + 1))
@@ -88,7 +95,7 @@ def test_dense_vs_oracle(n, n_out, S):
     exps[0] = 0
     nums = rng.integers(0, 32768, n).astype(np.uint16)
     got, valid, plain = run_dense(datas, lens, S, exps, nums=nums, split=n // 2 if n > 2 else None,
-                                  also_plain=True)
+                                  also_plain=True, form=form)
     inp = np.zeros((n, max(S, 1)), np.uint8)
     for k in range(n):
         inp[k, :lens[k]] = datas[k]
@@ -99,12 +106,13 @@ def test_dense_vs_oracle(n, n_out, S):
     assert (got == plain).all()
 
 
-def test_dense_config0_golden():
-    say("dense config0")
+@pytest.mark.parametrize("form", FORMS)
+def test_dense_config0_golden(form):
+    say(form, "config0")
     c = G["config0"]
     S, n = c["slice"], c["n"]
     datas = [O.prng_bytes(0, 0, i, S) for i in range(n)]
-    got, valid, _ = run_dense(datas, [S] * n, S, c["exps"])
+    got, valid, _ = run_dense(datas, [S] * n, S, c["exps"], form=form)
     off, _ = P.slot_layout(S, 64)
     for r in range(len(c["exps"])):
         assert hashlib.sha256(got[r, :S].tobytes()).hexdigest() == c["sha256"][r]
@@ -116,3 +124,28 @@ def test_dense_rejects_bad_shapes():
     buf = P.DeviceBuffer(4096)
     assert lib().pph_muladd_dense(buf.ptr, 1000, 1, buf.ptr, buf.ptr, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
     assert lib().pph_muladd_dense(buf.ptr, 1024, 1, None, None, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
+
+
+def test_bilinear_chunked(monkeypatch):
+    """Several coefficient blocks and word chunks (forced small) give the same bytes."""
+    say("bilinear chunked")
+    n, n_out, S = 700, 75, 10000
+    rng = np.random.default_rng(5)
+    datas = [O.prng_bytes(72, 0, k, S) for k in range(n)]
+    exps = rng.integers(0, 65535, n_out).astype(np.uint16)
+    full, _, plain = run_dense(datas, [S] * n, S, exps, also_plain=True, form="bilinear")
+    assert (full == plain).all()
+    lib().pph_bilinear_release()
+    monkeypatch.setenv("PPH_BILINEAR_ROWS", "32")
+    monkeypatch.setenv("PPH_BILINEAR_CHUNK_WORDS", "1024")
+    chunked, valid, _ = run_dense(datas, [S] * n, S, exps, form="bilinear")
+    assert (chunked == plain).all()
+    assert (valid == 1).all()
+    lib().pph_bilinear_release()
+
+
+def test_bilinear_rejects_bad_shapes():
+    buf = P.DeviceBuffer(4096)
+    assert lib().pph_muladd_bilinear(buf.ptr, 1000, 1, buf.ptr, buf.ptr, buf.ptr, 1, 0, None) == \
+        parpar_amd.PPH_E_INVALID
+    assert lib().pph_muladd_bilinear(buf.ptr, 1024, 1, None, None, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
