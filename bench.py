@@ -351,7 +351,7 @@ def main_batched(args):
         "ma_per_s": round(ma / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_muladd_batch (+ dense launches)", "kernel_ms": round(fused_ms, 3),
+                     "kernel": "k_muladd_batch (+ dense and bilinear launches)", "kernel_ms": round(fused_ms, 3),
                      "algorithmic_bytes": 2 * ma},
         "cpu_baseline": cpu,
         "autotune": tuning.as_dict(),

@@ -58,6 +58,7 @@ extern "C" {
 #define PPH_FORM_AUTO 0
 #define PPH_FORM_PLAIN 1   /* LDS log/exp streaming kernel */
 #define PPH_FORM_DENSE 2   /* GF(2) bit-matrix contraction on int8 MFMA */
+#define PPH_FORM_BILINEAR 3  /* rank-81 bilinear formula as batched int8 GEMMs (pph_muladd_bilinear) */
 
 const char* pph_strerror(int status);
 int pph_last_hip_error(void);
@@ -232,13 +233,18 @@ typedef struct pph_job {
 } pph_job;
 
 /* Measured mul-add rates of the forms by recovery-count bucket
- * (R <= 1, 4, 16, 64, 256, 1024, 4096, more), mul-adds per second. */
+ * (R <= 1, 4, 16, 64, 256, 1024, 4096, more), mul-adds per second, on 64 KiB
+ * slices: the plain and dense forms with 512 inputs, the bilinear form with
+ * 512 (bilinear) and 4096 inputs (bilinear_wide; the planner interpolates in
+ * log(inputs)).  A zero rate leaves the form out. */
 typedef struct pph_tuning {
 	unsigned version;
 	unsigned r_buckets;
 	float plain_g8[8];
 	float plain_g16[8];
 	float dense[8];
+	float bilinear[8];
+	float bilinear_wide[8];
 } pph_tuning;
 
 /* Measure the rates on the current device (about a second of GPU time). */
@@ -248,12 +254,15 @@ typedef struct pph_batch pph_batch;
 /* Plan a batch: tuning NULL runs (once per process) and uses pph_autotune. */
 int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned cksum_size, const pph_tuning* tuning,
                      pph_batch** out);
-/* The form (PPH_FORM_PLAIN / PPH_FORM_DENSE) chosen for a job, the plain
+/* The form (PPH_FORM_PLAIN / _DENSE / _BILINEAR) chosen for a job, the plain
  * kernel's output grouping and the predicted mul-add time of the batch. */
 int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group, double* predicted_s);
 /* Run the batch on a stream: counter reset, per-job preparation, the fused
- * persistent mul-add (and dense launches), per-job verification.  use_graph
- * captures that sequence into a hipGraph on first use and replays it. */
+ * persistent mul-add (and dense / bilinear launches), per-job verification.
+ * use_graph captures that sequence into a hipGraph on first use and replays
+ * it; the bilinear jobs' library GEMMs and their checks are enqueued after the
+ * replay, outside the graph.  A batch with bilinear jobs owns a device
+ * workspace for them (the largest job's pph_bilinear_workspace). */
 int pph_batch_run(pph_batch* b, int use_graph, void* stream);
 /* Time of the mul-add launches of the last eager (use_graph = 0) run, from HIP
  * events recorded on its stream around them; waits for that run. */

@@ -26,6 +26,7 @@ PPH_ADD_FULL = 2
 FORM_AUTO = 0
 FORM_PLAIN = 1
 FORM_DENSE = 2
+FORM_BILINEAR = 3
 
 _lib = None
 

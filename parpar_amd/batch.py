@@ -22,10 +22,12 @@ class PPHJob(ctypes.Structure):
 
 class PPHTuning(ctypes.Structure):
     _fields_ = [("version", ctypes.c_uint), ("r_buckets", ctypes.c_uint), ("plain_g8", ctypes.c_float * 8),
-                ("plain_g16", ctypes.c_float * 8), ("dense", ctypes.c_float * 8)]
+                ("plain_g16", ctypes.c_float * 8), ("dense", ctypes.c_float * 8), ("bilinear", ctypes.c_float * 8),
+                ("bilinear_wide", ctypes.c_float * 8)]
 
     def as_dict(self):
-        return {"plain_g8": list(self.plain_g8), "plain_g16": list(self.plain_g16), "dense": list(self.dense)}
+        return {"plain_g8": list(self.plain_g8), "plain_g16": list(self.plain_g16), "dense": list(self.dense),
+                "bilinear": list(self.bilinear), "bilinear_wide": list(self.bilinear_wide)}
 
 
 def autotune():
@@ -34,14 +36,16 @@ def autotune():
     return t
 
 
-def fixed_tuning(plain=1.0, dense=0.0, g16_over_g8=1.01):
+def fixed_tuning(plain=1.0, dense=0.0, bilinear=0.0, g16_over_g8=1.01):
     """A tuning table that makes the planner pick a given form (tests)."""
     t = PPHTuning()
-    t.version, t.r_buckets = 1, 8
+    t.version, t.r_buckets = 3, 8
     for b in range(8):
         t.plain_g8[b] = plain * 1e12
         t.plain_g16[b] = plain * 1e12 * g16_over_g8
         t.dense[b] = dense * 1e12
+        t.bilinear[b] = bilinear * 1e12
+        t.bilinear_wide[b] = bilinear * 1e12
     return t
 
 
@@ -99,7 +103,7 @@ class Batch:
         pred = ctypes.c_double()
         check(lib().pph_batch_plan(self._p, j, ctypes.byref(form), ctypes.byref(group), ctypes.byref(pred)),
               "pph_batch_plan")
-        return {"form": {1: "plain", 2: "dense"}.get(form.value, form.value), "group": group.value,
+        return {"form": {1: "plain", 2: "dense", 3: "bilinear"}.get(form.value, form.value), "group": group.value,
                 "predicted_s": pred.value}
 
     def run(self, use_graph=True, stream=None):

@@ -1,9 +1,9 @@
 // Batched execution: many independent PAR2 jobs (uneven input counts, slice
 // sizes and recovery counts) in one persistent fused mul-add launch, with the
 // per-job preparation and checksum verification around it captured once in a
-// hipGraph and replayed.  The form (plain or dense) and the plain kernel's
-// output grouping are chosen per batch from rates the autotuner measures on
-// the device.
+// hipGraph and replayed.  The form of each job (plain, dense bit-matrix or
+// bilinear) and the plain kernel's output grouping are chosen per batch from
+// rates the autotuner measures on the device.
 #include "pph_internal.h"
 #include "pph_plain_tile.h"
 #include "../../include/parpar_hip.h"
@@ -76,7 +76,7 @@ int autotune(pph_tuning* out) {
 	DeviceTables* t;
 	int rc = device_tables(&t);
 	if(rc) return rc;
-	const unsigned N = 512;
+	const unsigned N = 512, N_wide = 4096;
 	const size_t S = 65536;
 	const unsigned ck = 64;
 	const size_t slot = slot_size(S, ck);
@@ -95,16 +95,16 @@ int autotune(pph_tuning* out) {
 		if(e1) (void)hipEventDestroy(e1);
 		if(s) (void)hipStreamDestroy(s);
 	};
-	if(hipMalloc(&in_log, N * slot) != hipSuccess || hipMalloc(&in_dat, N * slot) != hipSuccess ||
-	   hipMalloc(&outb, (size_t)1 << 30) != hipSuccess || hipMalloc(&logs, N * 2) != hipSuccess ||
+	if(hipMalloc(&in_log, N * slot) != hipSuccess || hipMalloc(&in_dat, N_wide * slot) != hipSuccess ||
+	   hipMalloc(&outb, (size_t)1 << 30) != hipSuccess || hipMalloc(&logs, N_wide * 2) != hipSuccess ||
 	   hipMalloc(&exps, R_max * 2) != hipSuccess) {
 		cleanup();
 		return PPH_E_NOMEM;
 	}
-	std::vector<uint16_t> h(std::max(N, R_max));
+	std::vector<uint16_t> h(std::max(N_wide, R_max));
 	hipError_t e = hipSuccess;
-	for(unsigned i = 0; i < N; i++) h[i] = host_input_log((uint16_t)i);
-	if(e == hipSuccess) e = hipMemcpy(logs, h.data(), N * 2, hipMemcpyHostToDevice);
+	for(unsigned i = 0; i < N_wide; i++) h[i] = host_input_log((uint16_t)i);
+	if(e == hipSuccess) e = hipMemcpy(logs, h.data(), N_wide * 2, hipMemcpyHostToDevice);
 	for(unsigned r = 0; r < R_max; r++) h[r] = (uint16_t)r;
 	if(e == hipSuccess) e = hipMemcpy(exps, h.data(), R_max * 2, hipMemcpyHostToDevice);
 	if(e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -112,14 +112,15 @@ int autotune(pph_tuning* out) {
 	if(e == hipSuccess) e = hipEventCreate(&e1);
 	if(e == hipSuccess) e = launch_fill_random(in_dat, slot, N, S, 12345, 0, 0, s);
 	if(e == hipSuccess) e = launch_prep(in_log, in_dat, slot, N, nullptr, nullptr, S, ck_offset(S), ck, true, *t, s);
-	if(e == hipSuccess) e = launch_prep(in_dat, nullptr, slot, N, nullptr, nullptr, S, ck_offset(S), ck, false, *t, s);
+	if(e == hipSuccess) e = launch_fill_random(in_dat, slot, N_wide, S, 12345, 0, 0, s);
+	if(e == hipSuccess) e = launch_prep(in_dat, nullptr, slot, N_wide, nullptr, nullptr, S, ck_offset(S), ck, false, *t, s);
 	// The plain form runs as the persistent batched kernel: measure it that way,
 	// on J jobs that share the input slots, enough work items to fill the GPU.
 	BatchJob* d_jobs = nullptr;
 	BatchItem* d_items = nullptr;
 	unsigned* d_counter = nullptr;
 	const size_t out_cap = (size_t)1 << 30;
-	auto timed = [&](int form, unsigned R, float* ms, double* ma) -> hipError_t {
+	auto timed = [&](int form, unsigned R, float* ms, double* ma, unsigned n_in) -> hipError_t {
 		hipError_t r = hipSuccess;
 		const unsigned G = form == 0 ? 8 : 16;
 		const unsigned tile_words = kPlainThreads * kPlainWordsPerLane;
@@ -127,10 +128,10 @@ int autotune(pph_tuning* out) {
 		const unsigned per_job = cols * ((R + G - 1) / G);
 		unsigned J = (2048 + per_job - 1) / per_job;
 		const unsigned jmax = (unsigned)std::max<size_t>(1, out_cap / ((size_t)R * slot));
-		if(form == 2) J = 1;
+		if(form >= 2) J = 1;
 		J = std::min(J, jmax);
-		*ma = (double)J * N * R * (S / 2);
-		if(form != 2) {
+		*ma = (double)J * n_in * R * (S / 2);
+		if(form < 2) {  // plain: the persistent batched kernel
 			std::vector<BatchJob> hj(J);
 			std::vector<BatchItem> hi;
 			for(unsigned j = 0; j < J; j++) {
@@ -167,7 +168,15 @@ int autotune(pph_tuning* out) {
 		for(int rep = 0; rep < 2 && r == hipSuccess; rep++) {
 			r = hipEventRecord(e0, s);
 			if(r != hipSuccess) break;
-			r = launch_muladd_dense((const uint16_t*)in_dat, slot / 2, N, logs, exps, (uint16_t*)outb, R, false, *t, s);
+			if(form == 3) {
+				const int st = pph_muladd_bilinear(in_dat, slot, n_in, logs, exps, outb, R, 0, s);
+				if(st) {
+					*ma = 0;  // form unavailable: rate 0
+					return hipSuccess;
+				}
+			} else {
+				r = launch_muladd_dense((const uint16_t*)in_dat, slot / 2, N, logs, exps, (uint16_t*)outb, R, false, *t, s);
+			}
 			if(r == hipSuccess) r = hipEventRecord(e1, s);
 			if(r == hipSuccess) r = hipEventSynchronize(e1);
 			if(r == hipSuccess) r = hipEventElapsedTime(ms, e0, e1);
@@ -177,23 +186,28 @@ int autotune(pph_tuning* out) {
 	if(e == hipSuccess) e = hipMalloc(&d_counter, 256);
 	pph_tuning tu;
 	memset(&tu, 0, sizeof(tu));
-	tu.version = 1;
+	tu.version = 3;
 	tu.r_buckets = kTuneBuckets;
 	for(unsigned b = 0; b < kTuneBuckets && e == hipSuccess; b++) {
 		const unsigned R = kTuneR[b];
 		double ma = 0;
 		float ms = 0;
-		e = timed(0, R, &ms, &ma);
+		e = timed(0, R, &ms, &ma, N);
 		tu.plain_g8[b] = (float)(ma / (ms * 1e-3));
-		if(e == hipSuccess) e = timed(1, R, &ms, &ma);
+		if(e == hipSuccess) e = timed(1, R, &ms, &ma, N);
 		tu.plain_g16[b] = (float)(ma / (ms * 1e-3));
-		if(e == hipSuccess) e = timed(2, R, &ms, &ma);
+		if(e == hipSuccess) e = timed(2, R, &ms, &ma, N);
 		tu.dense[b] = (float)(ma / (ms * 1e-3));
+		if(e == hipSuccess) e = timed(3, R, &ms, &ma, N);
+		tu.bilinear[b] = ma > 0 ? (float)(ma / (ms * 1e-3)) : 0.0f;
+		if(e == hipSuccess) e = timed(3, R, &ms, &ma, N_wide);
+		tu.bilinear_wide[b] = ma > 0 ? (float)(ma / (ms * 1e-3)) : 0.0f;
 	}
 	if(d_jobs) (void)hipFree(d_jobs);
 	if(d_items) (void)hipFree(d_items);
 	if(d_counter) (void)hipFree(d_counter);
 	cleanup();
+	(void)pph_bilinear_release();
 	if(e != hipSuccess) return set_error(e);
 	*out = tu;
 	return PPH_OK;
@@ -226,7 +240,9 @@ struct pph_batch {
 	int device = 0;
 	unsigned ck = 64;
 	std::vector<pph_job> jobs;
-	std::vector<int> form;         // PPH_FORM_PLAIN / PPH_FORM_DENSE per job
+	std::vector<int> form;         // PPH_FORM_PLAIN / _DENSE / _BILINEAR per job
+	void* bl_ws = nullptr;         // workspace of the bilinear jobs
+	size_t bl_ws_bytes = 0;
 	unsigned group = 16;           // plain kernel output grouping
 	DeviceTables* tab = nullptr;
 	BatchJob* d_jobs = nullptr;
@@ -241,7 +257,27 @@ struct pph_batch {
 	bool timing = false;
 };
 
-static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
+// Bilinear jobs' mul-adds (library GEMMs, outside any graph) and, with
+// verify, their checks.
+static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s, bool verify) {
+	for(size_t j = 0; j < b->jobs.size(); j++) {
+		const pph_job& jb = b->jobs[j];
+		if(b->form[j] != PPH_FORM_BILINEAR || !jb.n_out) continue;
+		const int st = bilinear_enqueue(jb.in_slots, jb.slot_bytes, jb.n_in, jb.in_logs, jb.exps, jb.out_slots, jb.n_out,
+		                                false, b->bl_ws, b->bl_ws_bytes, *b->tab, s);
+		if(st) return hipErrorLaunchFailure;
+		if(verify) {
+			hipError_t e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size,
+			                             ck_offset(jb.slice_size), b->ck, jb.valid, *b->tab, s);
+			if(e != hipSuccess) return e;
+		}
+	}
+	return hipSuccess;
+}
+
+// The batch's device work.  in_graph: leave the bilinear jobs' mul-adds and
+// checks out (enqueue_bilinear runs them after the replay).
+static hipError_t enqueue_batch(pph_batch* b, hipStream_t s, bool in_graph) {
 	hipError_t e = hipSuccess;
 	if(b->n_items) {
 		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, b->d_counter, 0u);
@@ -270,11 +306,12 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
 		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, s);
 	}
+	if(e == hipSuccess && !in_graph) e = enqueue_bilinear(b, s, false);
 	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_b, s);
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
-		if(!jb.n_out) continue;
-		if(!jb.n_in) continue;
+		if(!jb.n_out || !jb.n_in) continue;
+		if(in_graph && b->form[j] == PPH_FORM_BILINEAR) continue;
 		e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size, ck_offset(jb.slice_size),
 		                  b->ck, jb.valid, *b->tab, s);
 	}
@@ -293,6 +330,7 @@ extern "C" void pph_batch_destroy(pph_batch* b) {
 	if(b->d_counter) (void)hipFree(b->d_counter);
 	if(b->ev_a) (void)hipEventDestroy(b->ev_a);
 	if(b->ev_b) (void)hipEventDestroy(b->ev_b);
+	if(b->bl_ws) (void)hipFree(b->bl_ws);
 	delete b;
 }
 
@@ -349,11 +387,35 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		const pph_job& jb = b->jobs[j];
 		if(!jb.n_in || !jb.n_out) continue;
 		const unsigned bk = tune_bucket(jb.n_out);
-		const double tp = ma_of(jb) / std::max(1.0f, b->group == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
-		const double td = ma_of(jb) / std::max(1.0f, tu.dense[bk]);
-		b->form[j] = td < tp ? PPH_FORM_DENSE : PPH_FORM_PLAIN;
-		pred[j] = std::min(tp, td);
+		const double ma = ma_of(jb);
+		const double tp = ma / std::max(1.0f, b->group == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
+		const double td = tu.dense[bk] > 0 ? ma / tu.dense[bk] : 1e30;
+		double rb = 0;
+		if(tu.version >= 3 && tu.bilinear[bk] > 0 && tu.bilinear_wide[bk] > 0) {
+			// log-linear in the input count between the two measured widths
+			const double f = std::min(1.0, std::max(0.0, (std::log2((double)jb.n_in) - 9.0) / 3.0));
+			rb = std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
+		}
+		const double tb = rb > 0 ? ma / rb : 1e30;
+		double best = tp;
+		if(td < best) {
+			best = td;
+			b->form[j] = PPH_FORM_DENSE;
+		}
+		if(tb < best && bilinear_prepare(jb.slot_bytes, jb.n_in, jb.n_out) == PPH_OK) {
+			best = tb;
+			b->form[j] = PPH_FORM_BILINEAR;
+			b->bl_ws_bytes = std::max(b->bl_ws_bytes, bilinear_workspace_bytes(jb.slot_bytes, jb.n_in, jb.n_out));
+		}
+		pred[j] = best;
 		b->predicted_s += pred[j];
+	}
+	if(b->bl_ws_bytes && hipMalloc(&b->bl_ws, b->bl_ws_bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		b->bl_ws = nullptr;
+		b->bl_ws_bytes = 0;
+		pph_batch_destroy(b);
+		return PPH_E_NOMEM;
 	}
 
 	// work items of the plain jobs, longest job first (largest first keeps the
@@ -410,7 +472,7 @@ extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
 	hipStream_t s = (hipStream_t)stream;
 	if(!use_graph) {
 		b->timing = true;
-		hipError_t e = enqueue_batch(b, s);
+		hipError_t e = enqueue_batch(b, s, false);
 		b->timing = false;
 		if(e != hipSuccess) return set_error(e);
 		return PPH_OK;
@@ -418,7 +480,7 @@ extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
 	if(!b->exec) {
 		hipError_t e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
 		if(e != hipSuccess) return set_error(e);
-		hipError_t ee = enqueue_batch(b, b->cap);
+		hipError_t ee = enqueue_batch(b, b->cap, true);
 		e = hipStreamEndCapture(b->cap, &b->graph);
 		if(ee != hipSuccess) return set_error(ee);
 		if(e != hipSuccess) return set_error(e);
@@ -426,6 +488,7 @@ extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
 		if(e != hipSuccess) return set_error(e);
 	}
 	hipError_t e = hipGraphLaunch(b->exec, s);
+	if(e == hipSuccess) e = enqueue_bilinear(b, s, true);
 	if(e != hipSuccess) return set_error(e);
 	return PPH_OK;
 }

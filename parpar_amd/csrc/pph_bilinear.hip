@@ -411,6 +411,96 @@ extern "C" int pph_bilinear_release(void) {
 	return PPH_OK;
 }
 
+namespace pph {
+
+size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+	return bl_shape(slot_bytes, n_in, n_out).total;
+}
+
+// Library handle, recombination table and the GEMM plans of a call's shapes.
+// Host work only: run it before a stream capture that encloses the enqueue.
+int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+	if(!n_in || !n_out || n_in > 32768 || slot_bytes % kSlotAlign) return PPH_E_INVALID;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return set_error(e);
+	BlDevice* d = bl_device(dev);
+	std::lock_guard<std::mutex> lk(d->mu);
+	if(!d->lt && hipblasLtCreate(&d->lt) != HIPBLAS_STATUS_SUCCESS) return PPH_E_DEVICE;
+	if(!d->d_wtab) {
+		e = hipMalloc((void**)&d->d_wtab, kBl * sizeof(uint16_t));
+		if(e == hipSuccess) e = hipMemcpy(d->d_wtab, bl_recombine_table(), kBl * sizeof(uint16_t), hipMemcpyHostToDevice);
+		if(e != hipSuccess) return set_error(e);
+	}
+	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
+	for(size_t r0 = 0; r0 < n_out; r0 += sh.rows) {
+		const size_t rows = std::min<size_t>(sh.rows, n_out - r0);
+		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
+			const size_t wc = std::min(sh.wc, sh.slot_words - w0);
+			if(!gemm_plan(d, wc, rows, sh.k_pad, sh.wc * sh.k_pad, rows * sh.k_pad)->ok) return PPH_E_UNSUPPORTED;
+		}
+	}
+	return PPH_OK;
+}
+
+// Enqueue one bilinear mul-add on s with a caller-owned workspace of at least
+// bilinear_workspace_bytes.  bilinear_prepare must have succeeded for the shape.
+int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                     const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
+                     const DeviceTables& tab, hipStream_t s) {
+	if(!n_out) return PPH_OK;
+	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
+	if(!ws || ws_bytes < sh.total) return PPH_E_INVALID;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return set_error(e);
+	BlDevice* d = bl_device(dev);
+	std::lock_guard<std::mutex> lk(d->mu);
+	if(!d->lt || !d->d_wtab) return PPH_E_STATE;
+	uint8_t* coef = static_cast<uint8_t*>(ws);
+	uint8_t* words = coef + sh.coef_bytes;
+	int32_t* g = reinterpret_cast<int32_t*>(words + sh.word_bytes);
+	void* ltws = words + sh.word_bytes + sh.g_bytes;
+	const int cus = tab.cus > 0 ? tab.cus : 256;
+	const int32_t alpha = 1, beta = 0;
+	for(size_t r0 = 0; r0 < n_out; r0 += sh.rows) {
+		const unsigned rows = (unsigned)std::min<size_t>(sh.rows, n_out - r0);
+		const size_t coef_plane = (size_t)rows * sh.k_pad;
+		{
+			const size_t items = sh.k_pad / 32 * rows;
+			const unsigned grid = (unsigned)std::min<size_t>((items + 255) / 256, (size_t)cus * 16);
+			hipLaunchKernelGGL(k_bl_coef_planes, dim3(grid), dim3(256), 0, s, coef, sh.k_pad, n_in, (unsigned)r0, rows,
+			                   input_logs, exps, tab.exp);
+			e = hipGetLastError();
+			if(e != hipSuccess) return set_error(e);
+		}
+		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
+			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // multiple of 64
+			const size_t tiles = sh.k_pad / kBlTileI * ((wc + kBlTileW - 1) / kBlTileW);
+			hipLaunchKernelGGL(k_bl_word_planes, dim3((unsigned)std::min<size_t>(tiles, (size_t)cus * 8)), dim3(256), 0,
+			                   s, words, sh.k_pad, wc, sh.wc, (const uint8_t*)in_slots, slot_bytes, n_in, w0,
+			                   sh.slot_words);
+			e = hipGetLastError();
+			if(e != hipSuccess) return set_error(e);
+			auto it = d->plans.find(std::make_tuple(wc, (size_t)rows, sh.k_pad, sh.wc * sh.k_pad, coef_plane));
+			if(it == d->plans.end() || !it->second.ok) return PPH_E_STATE;
+			const GemmPlan* p = &it->second;
+			if(hipblasLtMatmul(d->lt, p->desc, &alpha, words, p->a, coef, p->b, &beta, g, p->c, g, p->c, &p->algo, ltws,
+			                   p->ws, s) != HIPBLAS_STATUS_SUCCESS)
+				return PPH_E_DEVICE;
+			const size_t items = wc / 2 * rows;
+			hipLaunchKernelGGL(k_bl_epilogue, dim3((unsigned)std::min<size_t>((items + 255) / 256, (size_t)cus * 16)),
+			                   dim3(256), 0, s, g, wc, rows, (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0,
+			                   accumulate ? 1 : 0, d->d_wtab);
+			e = hipGetLastError();
+			if(e != hipSuccess) return set_error(e);
+		}
+	}
+	return PPH_OK;
+}
+
+}  // namespace pph
+
 extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                                    const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate,
                                    void* stream) {
@@ -423,68 +513,30 @@ extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsi
 	DeviceTables* tab = nullptr;
 	int rc = device_tables(&tab);
 	if(rc) return rc;
+	rc = bilinear_prepare(slot_bytes, n_in, n_out);
+	if(rc) return rc;
 	BlDevice* d = bl_device(dev);
-	std::lock_guard<std::mutex> lk(d->mu);
-	if(!d->lt && hipblasLtCreate(&d->lt) != HIPBLAS_STATUS_SUCCESS) return PPH_E_DEVICE;
-	if(!d->d_wtab) {
-		e = hipMalloc((void**)&d->d_wtab, kBl * sizeof(uint16_t));
-		if(e == hipSuccess) e = hipMemcpy(d->d_wtab, bl_recombine_table(), kBl * sizeof(uint16_t), hipMemcpyHostToDevice);
-		if(e != hipSuccess) return set_error(e);
+	const size_t need = bilinear_workspace_bytes(slot_bytes, n_in, n_out);
+	void* ws = nullptr;
+	{
+		std::lock_guard<std::mutex> lk(d->mu);
+		if(d->buf_bytes < need) {
+			if(d->buf) {
+				e = hipDeviceSynchronize();
+				if(e != hipSuccess) return set_error(e);
+				(void)hipFree(d->buf);
+				d->buf = nullptr;
+				d->buf_bytes = 0;
+			}
+			e = hipMalloc((void**)&d->buf, need);
+			if(e != hipSuccess) {
+				(void)hipGetLastError();
+				return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
+			}
+			d->buf_bytes = need;
+		}
+		ws = d->buf;
 	}
-	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
-	if(d->buf_bytes < sh.total) {
-		if(d->buf) {
-			e = hipDeviceSynchronize();
-			if(e != hipSuccess) return set_error(e);
-			(void)hipFree(d->buf);
-			d->buf = nullptr;
-			d->buf_bytes = 0;
-		}
-		e = hipMalloc((void**)&d->buf, sh.total);
-		if(e != hipSuccess) {
-			(void)hipGetLastError();
-			return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
-		}
-		d->buf_bytes = sh.total;
-	}
-	uint8_t* coef = d->buf;
-	uint8_t* words = coef + sh.coef_bytes;
-	int32_t* g = reinterpret_cast<int32_t*>(words + sh.word_bytes);
-	void* ltws = words + sh.word_bytes + sh.g_bytes;
-	hipStream_t s = (hipStream_t)stream;
-	const int cus = tab->cus > 0 ? tab->cus : 256;
-	const int32_t alpha = 1, beta = 0;
-	for(size_t r0 = 0; r0 < n_out; r0 += sh.rows) {
-		const unsigned rows = (unsigned)std::min<size_t>(sh.rows, n_out - r0);
-		const size_t coef_plane = (size_t)rows * sh.k_pad;
-		{
-			const size_t items = sh.k_pad / 32 * rows;
-			const unsigned grid = (unsigned)std::min<size_t>((items + 255) / 256, (size_t)cus * 16);
-			hipLaunchKernelGGL(k_bl_coef_planes, dim3(grid), dim3(256), 0, s, coef, sh.k_pad, n_in, (unsigned)r0, rows,
-			                   input_logs, exps, tab->exp);
-			e = hipGetLastError();
-			if(e != hipSuccess) return set_error(e);
-		}
-		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
-			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // multiple of 64
-			const size_t tiles = sh.k_pad / kBlTileI * ((wc + kBlTileW - 1) / kBlTileW);
-			hipLaunchKernelGGL(k_bl_word_planes, dim3((unsigned)std::min<size_t>(tiles, (size_t)cus * 8)), dim3(256), 0,
-			                   s, words, sh.k_pad, wc, sh.wc, (const uint8_t*)in_slots, slot_bytes, n_in, w0,
-			                   sh.slot_words);
-			e = hipGetLastError();
-			if(e != hipSuccess) return set_error(e);
-			GemmPlan* p = gemm_plan(d, wc, rows, sh.k_pad, sh.wc * sh.k_pad, coef_plane);
-			if(!p->ok) return PPH_E_UNSUPPORTED;
-			if(hipblasLtMatmul(d->lt, p->desc, &alpha, words, p->a, coef, p->b, &beta, g, p->c, g, p->c, &p->algo, ltws,
-			                   p->ws, s) != HIPBLAS_STATUS_SUCCESS)
-				return PPH_E_DEVICE;
-			const size_t items = wc / 2 * rows;
-			hipLaunchKernelGGL(k_bl_epilogue, dim3((unsigned)std::min<size_t>((items + 255) / 256, (size_t)cus * 16)),
-			                   dim3(256), 0, s, g, wc, rows, (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0,
-			                   accumulate, d->d_wtab);
-			e = hipGetLastError();
-			if(e != hipSuccess) return set_error(e);
-		}
-	}
-	return PPH_OK;
+	return bilinear_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, ws,
+	                        d->buf_bytes, *tab, (hipStream_t)stream);
 }

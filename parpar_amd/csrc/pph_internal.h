@@ -69,6 +69,12 @@ hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned cou
                            uint64_t file_size, uint8_t* md5_out, hipStream_t s);
 // Spans of file MD5 chains (n Md5Span records, device-resident), one wave each.
 hipError_t launch_file_md5_round(const void* spans, unsigned n, hipStream_t s);
+// Bilinear form (pph_bilinear.hip); int status codes.
+size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
+int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);
+int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                     const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
+                     const DeviceTables& tab, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 

@@ -61,17 +61,25 @@ def refs():
     return [expected(j, job) for j, job in enumerate(jobs)]
 
 
-@pytest.mark.parametrize("mode", ["plain", "dense", "auto"])
+@pytest.mark.parametrize("mode", ["plain", "dense", "bilinear", "mixed", "auto"])
 def test_batch_eager_and_graph(mode, refs):
     say("batch", mode)
     jobs = make_jobs()
     tuning = {"plain": B.fixed_tuning(plain=1.0, dense=0.001), "dense": B.fixed_tuning(plain=0.001, dense=1.0),
-              "auto": None}[mode]
+              "bilinear": B.fixed_tuning(plain=0.001, dense=0.002, bilinear=1.0), "auto": None}.get(mode)
+    if mode == "mixed":
+        # every form in one batch: plain for R <= 4, dense for 5..16, bilinear above
+        tuning = B.fixed_tuning(plain=1.0, dense=0.0, bilinear=0.0)
+        for b in range(8):
+            tuning.dense[b] = 2e12 if b == 2 else 1e9
+            tuning.bilinear[b] = tuning.bilinear_wide[b] = 3e12 if b >= 3 else 1e9
     batch = B.Batch(jobs, tuning=tuning)
     forms = [batch.plan(j)["form"] for j in range(len(jobs))]
     say(" forms", forms, "group", batch.plan(0)["group"])
-    if mode != "auto":
+    if mode in ("plain", "dense", "bilinear"):
         assert all(f == mode for f in forms)
+    if mode == "mixed":
+        assert set(forms) == {"plain", "dense", "bilinear"}, forms
 
     fill_all(jobs)
     batch.run(use_graph=False)
@@ -102,7 +110,7 @@ def test_autotune_rates_are_sane():
     t = B.autotune()
     d = t.as_dict()
     say("tuning", d)
-    for k in ("plain_g8", "plain_g16", "dense"):
+    for k in ("plain_g8", "plain_g16", "dense", "bilinear", "bilinear_wide"):
         assert all(v > 1e9 for v in d[k]), k
     # the plain form's rate grows with the recovery count until it is compute bound
     assert d["plain_g16"][4] > d["plain_g16"][0]
