@@ -117,12 +117,14 @@ def main_serving(args):
         len(arrivals), args.window, total_bytes / 2**30, total_ma))
     free, tot = ctypes.c_size_t(), ctypes.c_size_t()
     chk(L.pph_mem_get_info(ctypes.byref(free), ctypes.byref(tot)))
-    budget = min(free.value - (16 << 30), 240 << 30)
+    bl_ws = args.bilinear_workspace << 30
+    budget = min(free.value - (16 << 30) - bl_ws, 240 << 30)
     name = ctypes.create_string_buffer(256)
     cus = ctypes.c_int()
     L.pph_device_info(0, name, 256, ctypes.byref(cus))
-    srv = SV.Server(budget, max_compute=args.max_compute)
-    log("arena %.1f GiB, %d compute streams" % (budget / 2**30, args.max_compute))
+    srv = SV.Server(budget, max_compute=args.max_compute, bilinear_workspace=bl_ws)
+    log("arena %.1f GiB, %d compute streams, bilinear workspace %d GiB" % (
+        budget / 2**30, args.max_compute, args.bilinear_workspace))
     # warm-up job (tables, first launches), not counted
     srv.run_all([SV.FileJob(8 << 20, 1 << 20, 4, seed=args.seed, job_id=1 << 30)])
 
@@ -226,7 +228,7 @@ def main_serving(args):
         "config": {"workload": "serving: Poisson 20 files/s for %.0f s, file 1 MiB..8 GiB, slice 256 KiB..16 MiB, "
                                "recovery 1..30%% of slices" % args.window, "config_index": 4,
                    "files": len(arrivals), "file_bytes": total_bytes, "mul_adds": total_ma,
-                   "arena_bytes": budget, "compute_streams": args.max_compute,
+                   "arena_bytes": budget, "compute_streams": args.max_compute, "bilinear_workspace": bl_ws,
                    "device": name.value.decode(), "cus": cus.value},
         "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
@@ -237,7 +239,8 @@ def main_serving(args):
                     "unit": "GB/s of file bytes"},
         "achieved": {"bytes_per_s": round(total_bytes / makespan / 1e9, 3), "ma_per_s": total_ma / makespan,
                      "makespan_s": round(makespan, 2)},
-        "engine": {"peak_jobs": st["peak_jobs"], "peak_arena_bytes": st["peak_bytes"], "md5_rounds": st["md5_rounds"]},
+        "engine": {"peak_jobs": st["peak_jobs"], "peak_arena_bytes": st["peak_bytes"], "md5_rounds": st["md5_rounds"],
+                   "bilinear_jobs": st["bilinear_jobs"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole serving pipeline (mul-add-equivalent bytes over the makespan)"},
@@ -378,6 +381,8 @@ def parse():
     ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving"], default=None)
     ap.add_argument("--window", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
+    ap.add_argument("--bilinear-workspace", type=int, default=40,
+                    help="serving: GiB of device workspace for the bilinear form (0: plain only)")
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")

@@ -281,6 +281,10 @@ typedef struct pph_serve_config {
 	unsigned max_compute;    /* 1..64 jobs computing at once */
 	size_t md5_quantum;      /* bytes of each file's MD5 chain per hash round (multiple of 64; 0 = 8 MiB) */
 	size_t staging_bytes;    /* each of the two pinned staging chunks (0 = 64 MiB) */
+	size_t bilinear_workspace;  /* device bytes for the bilinear form, allocated at create beside the
+	                               budget (0: plain form only).  Jobs whose predicted bilinear time is
+	                               lower and whose workspace fits use it, one job's GEMMs at a time. */
+	const struct pph_tuning* tuning;  /* rates for that choice; NULL runs pph_autotune at create */
 } pph_serve_config;
 
 /* One file.  Its slices (slice_size bytes, the last one partial) are PAR2
@@ -320,6 +324,7 @@ typedef struct pph_serve_result {
 
 typedef struct pph_serve_stats {
 	uint64_t submitted, completed, md5_rounds;
+	uint64_t bilinear_jobs;     /* completed jobs that used the bilinear form */
 	unsigned peak_jobs;         /* most jobs admitted at once */
 	uint64_t peak_bytes;        /* most arena bytes in use at once */
 	uint64_t arena_bytes;

@@ -72,6 +72,19 @@ unsigned tune_bucket(unsigned r) {
 	return kTuneBuckets - 1;
 }
 
+double bilinear_rate(const pph_tuning& tu, unsigned n_in, unsigned n_out) {
+	const unsigned bk = tune_bucket(n_out);
+	if(tu.version < 3 || !(tu.bilinear[bk] > 0) || !(tu.bilinear_wide[bk] > 0) || !n_in) return 0;
+	// log-linear in the input count between the two measured widths (512, 4096)
+	const double f = std::min(1.0, std::max(0.0, (std::log2((double)n_in) - 9.0) / 3.0));
+	return std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
+}
+
+double plain_rate(const pph_tuning& tu, unsigned n_out) {
+	const unsigned bk = tune_bucket(n_out);
+	return std::max(1.0f, std::max(tu.plain_g8[bk], tu.plain_g16[bk]));
+}
+
 int autotune(pph_tuning* out) {
 	DeviceTables* t;
 	int rc = device_tables(&t);
@@ -390,12 +403,7 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		const double ma = ma_of(jb);
 		const double tp = ma / std::max(1.0f, b->group == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
 		const double td = tu.dense[bk] > 0 ? ma / tu.dense[bk] : 1e30;
-		double rb = 0;
-		if(tu.version >= 3 && tu.bilinear[bk] > 0 && tu.bilinear_wide[bk] > 0) {
-			// log-linear in the input count between the two measured widths
-			const double f = std::min(1.0, std::max(0.0, (std::log2((double)jb.n_in) - 9.0) / 3.0));
-			rb = std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
-		}
+		const double rb = bilinear_rate(tu, jb.n_in, jb.n_out);
 		const double tb = rb > 0 ? ma / rb : 1e30;
 		double best = tp;
 		if(td < best) {

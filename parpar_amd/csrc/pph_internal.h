@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+struct pph_tuning;
+
 namespace pph {
 
 // GF(2^16) with the PAR2 polynomial x^16 + x^12 + x^3 + x + 1 (0x1100B),
@@ -69,6 +71,11 @@ hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned cou
                            uint64_t file_size, uint8_t* md5_out, hipStream_t s);
 // Spans of file MD5 chains (n Md5Span records, device-resident), one wave each.
 hipError_t launch_file_md5_round(const void* spans, unsigned n, hipStream_t s);
+// Rate model of the autotuner's table (pph_batch.hip): mul-adds per second.
+double bilinear_rate(const struct ::pph_tuning& tu, unsigned n_in, unsigned n_out);  // 0: unavailable
+double plain_rate(const struct ::pph_tuning& tu, unsigned n_out);
+int autotune(struct ::pph_tuning* out);
+
 // Bilinear form (pph_bilinear.hip); int status codes.
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);

@@ -3,8 +3,13 @@
 // admits them while device memory (a fixed arena, sized by the caller) and a
 // compute stream are free, so new jobs start while earlier ones still run and
 // their kernels overlap on the device.  An admitted job runs on its own stream:
-//   inputs -> per-slice MD5/CRC-32 -> preparation (gf16_cksum, log domain, out
-//   of place) -> plain mul-add -> gf16_cksum check of the recovery slices
+//   inputs -> per-slice MD5/CRC-32 -> preparation (gf16_cksum, out of place)
+//   -> mul-add -> gf16_cksum check of the recovery slices
+// The mul-add is the plain form (log-domain preparation) or, for jobs where
+// the autotuned rates predict it faster, the bilinear form (data-form
+// preparation, library GEMMs in an engine-owned workspace; the bilinear
+// phases of different jobs are ordered through one event so that they take
+// the workspace one at a time).
 // Inputs come from the caller's host buffer through two pinned staging chunks
 // (double-buffered hipMemcpyAsync, on an upload thread), or from the
 // counter-based generator on the device.
@@ -104,6 +109,7 @@ struct Job {
 	int* valid = nullptr;
 	int stream = -1;
 	hipEvent_t ev_data = nullptr, ev_done = nullptr;
+	int form = PPH_FORM_PLAIN;
 	bool launched = false;   // inputs and compute enqueued (guarded by the engine mutex)
 	bool delivering = false; // handed to the delivery thread
 	bool computed = false;   // compute results copied back, B released (guarded by the engine mutex)
@@ -140,6 +146,11 @@ struct pph_serve {
 	unsigned next_round_buf = 0;
 	uint8_t* pinned[2] = {nullptr, nullptr};
 	hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+	void* bl_ws = nullptr;               // bilinear workspace
+	hipEvent_t bl_tail = nullptr;        // end of the last enqueued bilinear phase
+	bool bl_any = false;
+	std::mutex bl_mu;                    // orders the bilinear phases' enqueue
+	pph_tuning tuning{};
 
 	std::mutex mu;
 	std::condition_variable cv;
@@ -193,12 +204,26 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 	hipError_t e = hipEventRecord(j->ev_data, a);
 	if(e == hipSuccess)
 		e = launch_slice_hash(j->data, j->slot, (unsigned)j->n_in, j->lens, S, j->smd5, j->scrc, *s->tab, a);
+	if(e == hipSuccess && j->n_out && j->form == PPH_FORM_BILINEAR &&
+	   bilinear_prepare(j->slot, (unsigned)j->n_in, j->n_out) != PPH_OK)
+		j->form = PPH_FORM_PLAIN;
 	if(e == hipSuccess && j->n_out) {
-		e = launch_prep(j->logd, j->data, j->slot, (unsigned)j->n_in, j->lens, nullptr, S, ck_offset(S), s->ck, true,
+		const bool bl = j->form == PPH_FORM_BILINEAR;
+		e = launch_prep(j->logd, j->data, j->slot, (unsigned)j->n_in, j->lens, nullptr, S, ck_offset(S), s->ck, !bl,
 		                *s->tab, a);
-		if(e == hipSuccess)
+		if(e == hipSuccess && bl) {
+			std::lock_guard<std::mutex> lk(s->bl_mu);
+			if(s->bl_any) e = hipStreamWaitEvent(a, s->bl_tail, 0);
+			if(e == hipSuccess &&
+			   bilinear_enqueue(j->logd, j->slot, (unsigned)j->n_in, j->logs, j->d_exps, j->out, j->n_out, false, s->bl_ws,
+			                    s->cfg.bilinear_workspace, *s->tab, a) != PPH_OK)
+				e = hipErrorLaunchFailure;
+			if(e == hipSuccess) e = hipEventRecord(s->bl_tail, a);
+			if(e == hipSuccess) s->bl_any = true;
+		} else if(e == hipSuccess) {
 			e = launch_muladd_plain((const uint16_t*)j->logd, j->slot / 2, (unsigned)j->n_in, j->logs, j->d_exps,
 			                        nullptr, 0, (uint16_t*)j->out, j->n_out, false, 0, *s->tab, a);
+		}
 		if(e == hipSuccess) e = launch_verify(j->out, j->slot, j->n_out, S, ck_offset(S), s->ck, j->valid, *s->tab, a);
 	}
 	if(e == hipSuccess) e = hipEventRecord(j->ev_done, a);
@@ -348,6 +373,7 @@ void finish_job(pph_serve* s, Job* j) {
 	s->active.erase(std::find(s->active.begin(), s->active.end(), j));
 	s->done.push_back(r);
 	s->stats.completed++;
+	if(j->form == PPH_FORM_BILINEAR) s->stats.bilinear_jobs++;
 	if(r.status == PPH_OK) {
 		s->stats.mul_adds += r.mul_adds;
 		s->stats.bytes += r.bytes;
@@ -577,6 +603,8 @@ extern "C" void pph_serve_destroy(pph_serve* s) {
 		if(s->d_spans[k]) (void)hipFree(s->d_spans[k]);
 	}
 	if(s->arena_mem) (void)hipFree(s->arena_mem);
+	if(s->bl_tail) (void)hipEventDestroy(s->bl_tail);
+	if(s->bl_ws) (void)hipFree(s->bl_ws);
 	delete s;
 }
 
@@ -622,9 +650,24 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 		if(e == hipSuccess) e = hipEventCreateWithFlags(&s->round_ev[k], hipEventDisableTiming);
 	}
 	if(e == hipSuccess) e = hipStreamSynchronize(s->hash_stream);
+	if(e == hipSuccess && c.bilinear_workspace) {
+		e = hipMalloc(&s->bl_ws, c.bilinear_workspace);
+		if(e == hipSuccess) e = hipEventCreateWithFlags(&s->bl_tail, hipEventDisableTiming);
+		if(e == hipSuccess) {
+			if(c.tuning) {
+				s->tuning = *c.tuning;
+			} else {
+				rc = autotune(&s->tuning);
+				if(rc) {
+					pph_serve_destroy(s);
+					return rc;
+				}
+			}
+		}
+	}
 	if(e != hipSuccess) {
 		pph_serve_destroy(s);
-		return set_error(e);
+		return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
 	}
 	s->stats.arena_bytes = c.hbm_budget;
 	s->sched = std::thread(scheduler, s);
@@ -654,6 +697,14 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 			delete j;
 			return PPH_E_INVALID;
 		}
+	}
+	if(s->bl_ws && j->n_out) {
+		// form by predicted time, the bilinear form only where its workspace fits
+		const double ma = (double)j->n_in * j->n_out * (double)(j->slot / 2);
+		const double rb = bilinear_rate(s->tuning, (unsigned)j->n_in, j->n_out);
+		if(rb > 0 && ma / rb < ma / plain_rate(s->tuning, j->n_out) &&
+		   bilinear_workspace_bytes(j->slot, (unsigned)j->n_in, j->n_out) <= s->cfg.bilinear_workspace)
+			j->form = PPH_FORM_BILINEAR;
 	}
 	j->bytes_a = bytes_a(j->n_in, j->slot);
 	j->bytes_b = bytes_b(j->n_in, j->n_out, j->slot);

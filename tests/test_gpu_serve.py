@@ -12,6 +12,7 @@ import pytest
 
 import oracle as O
 import parpar_amd
+from parpar_amd import batch as B
 from parpar_amd import serve as SV
 from golden_io import expected_valid
 
@@ -113,6 +114,35 @@ def test_serve_alone_vs_mixed():
         assert (getattr(alone, name) == getattr(mixed, name)).all(), name
     for job in others:
         check_job(job)
+
+
+def test_serve_bilinear_form():
+    """Jobs routed to the bilinear form (forced by the rate table) give the
+    oracle's bytes, alone and mixed, and the same bytes as the plain form."""
+    say("serve bilinear")
+    force = B.fixed_tuning(plain=0.001, dense=0.0, bilinear=1.0)
+    jobs = mixed_jobs()
+    with SV.Server(hbm_budget=256 * MiB, max_compute=4, md5_quantum=64 * 1024, bilinear_workspace=12 << 30,
+                   tuning=force) as srv:
+        srv.run_all(jobs)
+        st = srv.stats()
+    say(" stats", st)
+    assert st["bilinear_jobs"] == sum(1 for j in jobs if j.n_out)
+    for job in jobs:
+        check_job(job)
+
+    def target():
+        return SV.FileJob(9 * 32768 + 123, 32768, 6, seed=11, job_id=77, first_input=3)
+
+    plain = target()
+    with SV.Server(hbm_budget=64 * MiB, max_compute=1) as srv:
+        srv.run_all([plain])
+    bl = target()
+    with SV.Server(hbm_budget=64 * MiB, max_compute=2, bilinear_workspace=4 << 30, tuning=force) as srv:
+        srv.run_all(mixed_jobs()[:4] + [bl])
+        assert srv.stats()["bilinear_jobs"] >= 1
+    for name in ("recovery", "valid", "slice_md5", "slice_crc", "file_md5"):
+        assert (getattr(plain, name) == getattr(bl, name)).all(), name
 
 
 @pytest.mark.parametrize("quantum", [64, 4096, 8 * MiB])
