@@ -86,7 +86,10 @@ int pph_proc_create(int device, unsigned staging_areas, pph_proc** out);
 /* PAR2ProcOCL::init (gf16/controller_ocl.cpp:67-90): cksum_size is the checksum
  * width of the CPU method the host selects (Galois16Mul::info().cksumSize,
  * gf16/gf16mul.cpp:215-409; 64 on AVX512+GFNI hosts); input_batch 0 picks the
- * default; form is a PPH_FORM_*.  setSliceSize must have been called. */
+ * default; form is PPH_FORM_AUTO / _PLAIN (plain kernel) or _BILINEAR (the
+ * bilinear form for batches numbered by input; worth it with large input
+ * batches and many recovery slices; explicit-coefficient batches still run
+ * the plain kernel).  setSliceSize must have been called. */
 int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_batch, int form);
 /* deinit / destructor (controller.h:198-203, controller_ocl.cpp:150-153) */
 void pph_proc_destroy(pph_proc* p);
@@ -188,7 +191,8 @@ int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, con
  * parity/recombination kernels; 81 MACs per mul-add instead of 256.  Needs a
  * device workspace (pph_bilinear_workspace bytes), allocated on first use,
  * kept for later calls and freed by pph_bilinear_release.  Calls on one device
- * are serialised. */
+ * share it: each call's device work starts after the previous call's ends,
+ * whatever streams they use. */
 int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                         const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes);

@@ -522,10 +522,13 @@ int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
 	uint16_t* logs;
 	meta_ptrs(p, a, &lens, &slots, &logs);
 	const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
+	// the bilinear form takes data-form inputs; batches with explicit
+	// coefficients always run the plain form
+	const bool bl = p->form == PPH_FORM_BILINEAR && !a.explicit_coef && !p->exps.empty();
 	PPH_HIP(hipMemcpyAsync(a.d_meta, a.h_meta, meta, hipMemcpyHostToDevice, p->s_copy));
 	if(a.prep_count) {
 		PPH_HIP(launch_prep(a.d_in, nullptr, p->slot_bytes, a.prep_count, a.d_meta, a.d_meta + p->batch, p->cur_size,
-		                    ck_offset(p->cur_size), p->ck, true, *p->tab, p->s_copy));
+		                    ck_offset(p->cur_size), p->ck, !bl, *p->tab, p->s_copy));
 		a.prep_count = 0;
 	}
 	if(a.explicit_coef) {
@@ -534,7 +537,13 @@ int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
 	}
 	PPH_HIP(hipEventRecord(p->ev_copied, p->s_copy));
 	PPH_HIP(hipStreamWaitEvent(p->s_comp, p->ev_copied, 0));
-	if(!p->exps.empty()) {
+	if(bl) {
+		const uint16_t* d_logs = reinterpret_cast<const uint16_t*>(a.d_meta + 2 * p->batch);
+		const int rc = pph_muladd_bilinear(a.d_in, p->slot_bytes, n, d_logs, p->d_exps, p->d_out,
+		                                   (unsigned)p->exps.size(), p->processing_add ? 1 : 0, p->s_comp);
+		if(rc) return rc;
+		p->processing_add = true;
+	} else if(!p->exps.empty()) {
 		const uint16_t* d_logs = reinterpret_cast<const uint16_t*>(a.d_meta + 2 * p->batch);
 		hipError_t e = launch_muladd_plain(reinterpret_cast<const uint16_t*>(a.d_in), p->slot_bytes / 2, n, d_logs,
 		                                   p->d_exps, a.explicit_coef ? a.d_coef : nullptr, p->r_pad, (uint16_t*)p->d_out,
@@ -687,7 +696,7 @@ extern "C" int pph_proc_set_slice_size(pph_proc* p, size_t slice_size) {
 
 extern "C" int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_batch, int form) {
 	if(!p || !valid_ck(cksum_size)) return PPH_E_INVALID;
-	if(form != PPH_FORM_AUTO && form != PPH_FORM_PLAIN) return PPH_E_UNSUPPORTED;
+	if(form != PPH_FORM_AUTO && form != PPH_FORM_PLAIN && form != PPH_FORM_BILINEAR) return PPH_E_UNSUPPORTED;
 	PPH_HIP(hipSetDevice(p->device));
 	if(p->s_comp) PPH_HIP(hipStreamSynchronize(p->s_comp));
 	p->ck = cksum_size;
@@ -721,8 +730,13 @@ extern "C" int pph_proc_set_current_slice_size(pph_proc* p, size_t slice_size) {
 	p->cur_size = slice_size;
 	p->slot_bytes = slot;
 	if(slot > p->alloc_slot_bytes || slice_size > p->alloc_slice) {
-		// grow (controller_ocl.cpp:227-240 re-inits in that case)
+		// grow: PAR2ProcOCL re-inits (controller_ocl.cpp:211-222), and init
+		// starts the recovery afresh (processingAdd = false, :147)
 		p->slice_size = std::max(p->slice_size, slice_size);
+		p->processing_add = false;
+		p->verified = false;
+		p->cur_area = p->cur_inputs = 0;
+		p->active_count = 0;
 		int rc = alloc_areas(p);
 		if(rc) return rc;
 		rc = ensure_coef_tables(p);
@@ -807,8 +821,8 @@ extern "C" int pph_proc_fill_input(pph_proc* p, const void* buffer) {
 	memcpy(stage, buffer, p->cur_size);
 	uint8_t* dst = a.d_in + (size_t)p->cur_inputs * p->slot_bytes;
 	PPH_HIP(hipMemcpyAsync(dst, stage, p->cur_size, hipMemcpyHostToDevice, p->s_copy));
-	PPH_HIP(launch_prep(dst, nullptr, p->slot_bytes, 1, nullptr, nullptr, p->cur_size, ck_offset(p->cur_size), p->ck, true,
-	                    *p->tab, p->s_copy));
+	PPH_HIP(launch_prep(dst, nullptr, p->slot_bytes, 1, nullptr, nullptr, p->cur_size, ck_offset(p->cur_size), p->ck,
+	                    p->form != PPH_FORM_BILINEAR, *p->tab, p->s_copy));
 	PPH_HIP(hipStreamSynchronize(p->s_copy));
 	if(++p->cur_inputs == p->batch) {
 		p->cur_inputs = 0;

@@ -282,6 +282,9 @@ struct BlDevice {
 	uint8_t* buf = nullptr;  // workspace: coefficient planes | word planes | G | library workspace
 	size_t buf_bytes = 0;
 	uint16_t* d_wtab = nullptr;
+	hipEvent_t tail = nullptr;  // end of the last call on the shared workspace
+	bool tail_valid = false;
+	std::mutex call_mu;         // held across one call's wait, enqueue and record
 	std::map<std::tuple<size_t, size_t, size_t, size_t, size_t>, GemmPlan> plans;  // (M, N, K, strides)
 };
 
@@ -516,6 +519,7 @@ extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsi
 	rc = bilinear_prepare(slot_bytes, n_in, n_out);
 	if(rc) return rc;
 	BlDevice* d = bl_device(dev);
+	std::lock_guard<std::mutex> call(d->call_mu);
 	const size_t need = bilinear_workspace_bytes(slot_bytes, n_in, n_out);
 	void* ws = nullptr;
 	{
@@ -536,7 +540,21 @@ extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsi
 			d->buf_bytes = need;
 		}
 		ws = d->buf;
+		if(!d->tail) {
+			e = hipEventCreateWithFlags(&d->tail, hipEventDisableTiming);
+			if(e != hipSuccess) return set_error(e);
+		}
+		// calls from different streams share the workspace: each waits for the last
+		if(d->tail_valid) {
+			e = hipStreamWaitEvent((hipStream_t)stream, d->tail, 0);
+			if(e != hipSuccess) return set_error(e);
+		}
 	}
-	return bilinear_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, ws,
-	                        d->buf_bytes, *tab, (hipStream_t)stream);
+	rc = bilinear_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, ws,
+	                      d->buf_bytes, *tab, (hipStream_t)stream);
+	std::lock_guard<std::mutex> lk(d->mu);
+	e = hipEventRecord(d->tail, (hipStream_t)stream);
+	if(e != hipSuccess) return set_error(e);
+	d->tail_valid = true;
+	return rc;
 }

@@ -22,14 +22,17 @@ def say(*a):
     print(*a, flush=True)
 
 
-def new_backend(slice_size, batch, areas=2, ck=64):
+FORMS = {"plain": parpar_amd.FORM_PLAIN, "bilinear": parpar_amd.FORM_BILINEAR}
+
+
+def new_backend(slice_size, batch, areas=2, ck=64, form="plain"):
     be = P.PAR2ProcHIP(0, areas)
     be.setSliceSize(slice_size)
-    be.init(cksum_size=ck, input_batch=batch)
+    be.init(cksum_size=ck, input_batch=batch, form=FORMS[form])
     return be
 
 
-def run_case(case, batch, areas):
+def run_case(case, batch, areas, form="plain"):
     inp = case_inputs(case)
     S = case["S"]
     c = P.PAR2Proc()
@@ -45,7 +48,7 @@ def run_case(case, batch, areas):
         allocs.append((be, off, size))
     assert c.init(S, allocs)
     for be, off, size in allocs:
-        be.init(cksum_size=64, input_batch=batch)
+        be.init(cksum_size=64, input_batch=batch, form=FORMS[form])
     assert c.setRecoverySlices(np.array(case["exps"], np.uint16))
     coeffs = np.array(case["coeffs"], np.uint16) if case["coeffs"] is not None else None
     for k in range(case["n"]):
@@ -67,13 +70,14 @@ def run_case(case, batch, areas):
     return outs, valid
 
 
-@pytest.mark.parametrize("batch,areas", [(0, 2), (3, 2), (1, 3), (5, 1)])
-def test_golden_small_cases(batch, areas):
-    say("small cases batch", batch, "areas", areas)
+@pytest.mark.parametrize("batch,areas,form", [(0, 2, "plain"), (3, 2, "plain"), (1, 3, "plain"), (5, 1, "plain"),
+                                              (0, 2, "bilinear"), (3, 2, "bilinear")])
+def test_golden_small_cases(batch, areas, form):
+    say("small cases batch", batch, "areas", areas, form)
     outs = small_outputs()
     for case in G["small_cases"]:
         say(" case", case["id"], "n", case["n"], "S", case["S"], "split", case["split"])
-        got, valid = run_case(case, batch, areas)
+        got, valid = run_case(case, batch, areas, form)
         exp = outs["out_%d" % case["id"]]
         assert (got == exp).all(), case["id"]
         # checksum flags equal PAR2ProcCPU's, including its odd sub-slice case
@@ -81,11 +85,12 @@ def test_golden_small_cases(batch, areas):
         assert list(valid) == list(case["valid"]), case["id"]
 
 
-def test_config0_backend_golden():
-    say("config0 through PAR2ProcHIP")
+@pytest.mark.parametrize("form", ["plain", "bilinear"])
+def test_config0_backend_golden(form):
+    say("config0 through PAR2ProcHIP", form)
     c = G["config0"]
     S, n = c["slice"], c["n"]
-    be = new_backend(S, 16)
+    be = new_backend(S, 16 if form == "plain" else 64, form=form)
     be.setRecoverySlices(np.array(c["exps"], np.uint16))
     for i in range(n):
         if be.canAdd() == parpar_amd.PPH_ADD_FULL:
@@ -166,10 +171,11 @@ def test_current_slice_size_and_default_exponent():
     be.close()
 
 
-def test_fill_and_dummy_inputs():
-    say("fillInput / dummyInput")
+@pytest.mark.parametrize("form", ["plain", "bilinear"])
+def test_fill_and_dummy_inputs(form):
+    say("fillInput / dummyInput", form)
     S = 4096
-    be = new_backend(S, 2, areas=2)
+    be = new_backend(S, 2, areas=2, form=form)
     be.setRecoverySlices(np.array([1, 7], np.uint16))
     data = [O.prng_bytes(24, 0, k, S) for k in range(4)]
     done = False
