@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("PPH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_LIB = "/opt/rocm/lib"
-HIP_SOURCES = ["pph_kernels.hip", "pph_dense.hip", "pph_batch.hip", "pph_hash.hip", "pph_serve.hip", "pph_bilinear.hip",
+HIP_SOURCES = ["pph_kernels.hip", "pph_dense.hip", "pph_batch.hip", "pph_hash.hip", "pph_serve.hip", "pph_bilinear.hip", "pph_bl_mfma.hip",
                "pph_api.hip"]
 CXX_SOURCES = ["controller_hip.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

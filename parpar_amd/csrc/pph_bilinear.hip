@@ -12,7 +12,9 @@
 //   G_t[r][w] = sum_i Ac_t[r][i] Xw_t[w][i],
 // 81 MACs per GF(2^16) mul-add instead of the 256 of the 16x16 bit-matrix.
 //
-// Device pipeline per call (one stream):
+// Two implementations of the 81 GEMMs: the default hand-written fp4 MFMA
+// kernel over bit-packed planes (pph_bl_mfma.hip), and 81 batched int8 GEMMs in
+// hipBLASLt (PPH_BILINEAR_GEMM=library), whose pipeline is:
 //   k_bl_coef_planes   Ac_t[r][i] = E(c_ri)_t as int8 0/1 (coefficients from the
 //                      PAR2 input logs and recovery exponents)
 //   for each chunk of slot words:
@@ -24,20 +26,20 @@
 // so each Karatsuba level is a handful of XORs on whole words.
 #include "pph_internal.h"
 #include "pph_plain_tile.h"
+#include "pph_karatsuba.h"
 #include "../../include/parpar_hip.h"
 
 #include <hipblaslt/hipblaslt.h>
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
 #include <vector>
 
 namespace pph {
-
-constexpr unsigned kBl = 81;
 
 // ------------------------------------------------------------------ host maps
 
@@ -84,40 +86,6 @@ static const uint16_t* bl_recombine_table() {
 }
 
 // ------------------------------------------------------------------ device
-
-// Bit-sliced expansion: s[b] holds bit b of 32 values (bit k = value k); e[t]
-// receives component t of the same 32 values.  Same order as bl_expand_host.
-__device__ __forceinline__ void bl_expand_slices(const uint32_t s[16], uint32_t e[kBl]) {
-	uint32_t l1[24], l2[36], l3[54];
-#pragma unroll
-	for(int b = 0; b < 8; b++) {
-		l1[b] = s[b];
-		l1[8 + b] = s[8 + b];
-		l1[16 + b] = s[b] ^ s[8 + b];
-	}
-#pragma unroll
-	for(int k = 0; k < 3; k++)
-#pragma unroll
-		for(int b = 0; b < 4; b++) {
-			l2[12 * k + b] = l1[8 * k + b];
-			l2[12 * k + 4 + b] = l1[8 * k + 4 + b];
-			l2[12 * k + 8 + b] = l1[8 * k + b] ^ l1[8 * k + 4 + b];
-		}
-#pragma unroll
-	for(int k = 0; k < 9; k++)
-#pragma unroll
-		for(int b = 0; b < 2; b++) {
-			l3[6 * k + b] = l2[4 * k + b];
-			l3[6 * k + 2 + b] = l2[4 * k + 2 + b];
-			l3[6 * k + 4 + b] = l2[4 * k + b] ^ l2[4 * k + 2 + b];
-		}
-#pragma unroll
-	for(int k = 0; k < 27; k++) {
-		e[3 * k] = l3[2 * k];
-		e[3 * k + 1] = l3[2 * k + 1];
-		e[3 * k + 2] = l3[2 * k] ^ l3[2 * k + 1];
-	}
-}
 
 // four mask bits -> four 0/1 bytes
 __device__ __forceinline__ uint32_t bytes4(uint32_t nib) { return ((nib & 0xFu) * 0x00204081u) & 0x01010101u; }
@@ -416,8 +384,14 @@ extern "C" int pph_bilinear_release(void) {
 
 namespace pph {
 
+// PPH_BILINEAR_GEMM=library selects the hipBLASLt implementation.
+bool bl_library() {
+	const char* v = getenv("PPH_BILINEAR_GEMM");
+	return v && !strcmp(v, "library");
+}
+
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out) {
-	return bl_shape(slot_bytes, n_in, n_out).total;
+	return bl_library() ? bl_shape(slot_bytes, n_in, n_out).total : blm_workspace_bytes(slot_bytes, n_in, n_out);
 }
 
 // Library handle, recombination table and the GEMM plans of a call's shapes.
@@ -435,6 +409,7 @@ int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 		if(e == hipSuccess) e = hipMemcpy(d->d_wtab, bl_recombine_table(), kBl * sizeof(uint16_t), hipMemcpyHostToDevice);
 		if(e != hipSuccess) return set_error(e);
 	}
+	if(!bl_library()) return PPH_OK;
 	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.rows) {
 		const size_t rows = std::min<size_t>(sh.rows, n_out - r0);
@@ -452,12 +427,17 @@ int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, con
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                      const DeviceTables& tab, hipStream_t s) {
 	if(!n_out) return PPH_OK;
-	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
-	if(!ws || ws_bytes < sh.total) return PPH_E_INVALID;
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if(e != hipSuccess) return set_error(e);
 	BlDevice* d = bl_device(dev);
+	if(!bl_library()) {
+		if(!d->d_wtab) return PPH_E_STATE;
+		return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate, ws, ws_bytes,
+		                   d->d_wtab, tab, s);
+	}
+	const Shape sh = bl_shape(slot_bytes, n_in, n_out);
+	if(!ws || ws_bytes < sh.total) return PPH_E_INVALID;
 	std::lock_guard<std::mutex> lk(d->mu);
 	if(!d->lt || !d->d_wtab) return PPH_E_STATE;
 	uint8_t* coef = static_cast<uint8_t*>(ws);

@@ -1,0 +1,329 @@
+// Bilinear form on fp4 MFMA: the 81 integer GEMMs of the rank-81 formula
+// (see pph_bilinear.hip) computed by a hand-written kernel on
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (0 / 1.0, unit scales;
+// the fp32 sums are exact integers up to 2^24), twice the int8 rate.
+//
+// Operands are bit-packed planes, one bit per element:
+//   Cb[t][kw][r]  bit of E(c(r, 32kw + k))_t     (coefficients, outputs r)
+//   Wb[t][kw][w]  bit of E(x_{32kw + k}[w])_t     (input words w)
+// with element k of a 32-input group at bit 4(k % 8) + k / 8, so that nibble
+// dword q of the fp4 expansion is ((m >> q) & 0x11111111) << 1.  The GEMM
+// kernel (one workgroup per 256 words x 256 outputs x component t) expands
+// them into LDS (double-buffered, one 64-input step per iteration), runs 16
+// MFMAs per wave per step and writes the parity of each sum as a bit:
+//   P[t][r][w / 32]  bit w % 32 = parity of sum_i E(c_ri)_t E(x_i[w])_t
+// k_blm_combine recombines the 81 parity bits of each word into the 16-bit
+// recovery word (bit-sliced over 32 words) and stores or XORs it.
+#include "pph_internal.h"
+#include "pph_plain_tile.h"
+#include "pph_karatsuba.h"
+#include "../../include/parpar_hip.h"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace pph {
+
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// bit position of element k (0..31) of a 32-input group in a packed mask
+__device__ __forceinline__ unsigned blm_pos(unsigned k) { return 4 * (k & 7) + (k >> 3); }
+
+// 32 packed elements -> 16 bytes of e2m1 nibbles (0 or 1.0 = 0b0010)
+__device__ __forceinline__ uint4 blm_nibbles(uint32_t m) {
+	return make_uint4((m & 0x11111111u) << 1, ((m >> 1) & 0x11111111u) << 1, ((m >> 2) & 0x11111111u) << 1,
+	                  ((m >> 3) & 0x11111111u) << 1);
+}
+
+// Coefficient planes for outputs r0 .. r0 + n_rows (rows padded to r_pad with
+// zeros).  Thread: one output row r, one 32-input group kw; r varies fastest.
+__global__ __launch_bounds__(256) void k_blm_coef(uint32_t* __restrict__ cb, size_t kw_count, size_t r_pad,
+                                                  unsigned n_in, unsigned r0, unsigned n_rows,
+                                                  const uint16_t* __restrict__ in_logs,
+                                                  const uint16_t* __restrict__ exps,
+                                                  const uint16_t* __restrict__ exp_tab) {
+	const size_t total = kw_count * r_pad;
+	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const size_t kw = g / r_pad;
+		const unsigned r = (unsigned)(g - kw * r_pad);
+		uint32_t s[16];
+#pragma unroll
+		for(int b = 0; b < 16; b++) s[b] = 0;
+		if(r < n_rows) {
+			const uint32_t er = exps[r0 + r];
+			for(unsigned k = 0; k < 32; k++) {
+				const size_t i = kw * 32 + k;
+				if(i >= n_in) break;
+				uint32_t l = mod65535((uint32_t)in_logs[i] * er);
+				const uint32_t c = exp_tab[l == kOrder ? 0u : l];
+				const unsigned p = blm_pos(k);
+#pragma unroll
+				for(int b = 0; b < 16; b++) s[b] |= ((c >> b) & 1u) << p;
+			}
+		}
+		uint32_t e[kBl];
+		bl_expand_slices(s, e);
+#pragma unroll
+		for(unsigned t = 0; t < kBl; t++) cb[((size_t)t * kw_count + kw) * r_pad + r] = e[t];
+	}
+}
+
+// Word planes for words [w0, w0 + wc) of every input slot (rows padded to
+// wc_pad with zeros).  Thread: one word, one 32-input group; words vary fastest
+// so each of the 32 slot reads is coalesced.
+__global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits, size_t kw_count, size_t wc_pad,
+                                                   size_t wc, const uint8_t* __restrict__ in_slots, size_t slot_bytes,
+                                                   unsigned n_in, size_t w0) {
+	const size_t total = kw_count * wc_pad;
+	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const size_t kw = g / wc_pad;
+		const size_t wl = g - kw * wc_pad;
+		uint32_t s[16];
+#pragma unroll
+		for(int b = 0; b < 16; b++) s[b] = 0;
+		if(wl < wc) {
+			for(unsigned k = 0; k < 32; k++) {
+				const size_t i = kw * 32 + k;
+				if(i >= n_in) break;
+				const uint32_t x = reinterpret_cast<const uint16_t*>(in_slots + i * slot_bytes)[w0 + wl];
+				const unsigned p = blm_pos(k);
+#pragma unroll
+				for(int b = 0; b < 16; b++) s[b] |= ((x >> b) & 1u) << p;
+			}
+		}
+		uint32_t e[kBl];
+		bl_expand_slices(s, e);
+#pragma unroll
+		for(unsigned t = 0; t < kBl; t++) wbits[((size_t)t * kw_count + kw) * wc_pad + wl] = e[t];
+	}
+}
+
+constexpr unsigned kBlmTile = 256;  // words and outputs per workgroup
+constexpr unsigned kBlmRow = 32;    // LDS bytes per row and 64-input step
+
+// One workgroup: component t, words w0 .. w0+255, outputs r0 .. r0+255.
+// Waves 2x2, each 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
+// kw_count (32-input groups) is a multiple of 8: the loop runs whole groups of
+// four 64-input steps.
+__global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ wbits, const uint32_t* __restrict__ cb,
+                                                  uint32_t* __restrict__ par, size_t kw_count, size_t wc_pad,
+                                                  size_t r_pad, unsigned w_tiles, unsigned r_tiles) {
+	__shared__ __align__(16) uint8_t sA[2][kBlmTile * kBlmRow];
+	__shared__ __align__(16) uint8_t sB[2][kBlmTile * kBlmRow];
+	const unsigned tiles = w_tiles * r_tiles;
+	const unsigned t = blockIdx.x / tiles;
+	const unsigned tile = blockIdx.x - t * tiles;
+	const unsigned wt = tile % w_tiles, rt = tile / w_tiles;
+	const size_t w0 = (size_t)wt * kBlmTile, r0 = (size_t)rt * kBlmTile;
+	const uint32_t* A = wbits + (size_t)t * kw_count * wc_pad + w0;
+	const uint32_t* B = cb + (size_t)t * kw_count * r_pad + r0;
+	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const unsigned wr = wave >> 1, wcol = wave & 1;
+	const unsigned half = lane >> 5, row32 = lane & 31;
+
+	v16f acc[4][4];
+#pragma unroll
+	for(int i = 0; i < 4; i++)
+#pragma unroll
+		for(int j = 0; j < 4; j++)
+#pragma unroll
+			for(int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
+
+	auto stage = [&](unsigned buf, const uint32_t* m) {
+		uint4* da = reinterpret_cast<uint4*>(sA[buf] + tid * kBlmRow);
+		uint4* db = reinterpret_cast<uint4*>(sB[buf] + tid * kBlmRow);
+		da[0] = blm_nibbles(m[0]);
+		da[1] = blm_nibbles(m[1]);
+		db[0] = blm_nibbles(m[2]);
+		db[1] = blm_nibbles(m[3]);
+	};
+	const unsigned steps = (unsigned)(kw_count / 2);
+	// masks of step s: A groups 2s, 2s+1 of this thread's word row, B groups of
+	// its output row; fetched kDepth - 1 steps ahead into rotating register sets
+	constexpr unsigned kDepth = 4;
+	uint32_t pf[kDepth][4];
+	// 32-bit element offsets from uniform bases (a plane holds < 2^32 masks)
+	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
+	auto fetch = [&](uint32_t* m, unsigned st) {
+		const unsigned ia = 2 * st * lda + tid, ib = 2 * st * ldb + tid;
+		m[0] = A[ia];
+		m[1] = A[ia + lda];
+		m[2] = B[ib];
+		m[3] = B[ib + ldb];
+	};
+#pragma unroll
+	for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], k);  // steps >= kDepth
+	stage(0, pf[0]);
+	__syncthreads();
+	for(unsigned s0 = 0; s0 < steps; s0 += kDepth) {
+#pragma unroll
+		for(unsigned k = 0; k < kDepth; k++) {
+			const unsigned s = s0 + k;  // steps is a multiple of kDepth
+			const unsigned buf = s & 1;
+			v8i fa[4], fb[4];
+#pragma unroll
+			for(int i = 0; i < 4; i++) {
+				const uint4 qa =
+				    *reinterpret_cast<const uint4*>(sA[buf] + (128 * wr + 32 * i + row32) * kBlmRow + 16 * half);
+				const uint4 qb =
+				    *reinterpret_cast<const uint4*>(sB[buf] + (128 * wcol + 32 * i + row32) * kBlmRow + 16 * half);
+				fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
+				fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
+			}
+#pragma unroll
+			for(int i = 0; i < 4; i++)
+#pragma unroll
+				for(int j = 0; j < 4; j++)
+					acc[i][j] =
+					    __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 4, 4, 0, 127, 0, 127);
+			// step s + 1 into the other buffer (its masks arrived kDepth - 1 steps ago),
+			// then the fetch of step s + kDepth into the set step s used.  Both run
+			// unconditionally so that the load counters stay exact: past the end the
+			// fetch repeats the last step (in bounds) and the staged copy is never read.
+			stage(buf ^ 1, pf[(k + 1) % kDepth]);
+			fetch(pf[k], s + kDepth < steps ? s + kDepth : steps - 1);
+			__syncthreads();
+		}
+	}
+	// parity bits: lane (half h, row32 = output column) holds words 8(v/4) + 4h + v%4
+	const size_t wbk = wc_pad / 32;
+#pragma unroll
+	for(int i = 0; i < 4; i++)
+#pragma unroll
+		for(int j = 0; j < 4; j++) {
+			uint32_t m = 0;
+#pragma unroll
+			for(int v = 0; v < 16; v++)
+				m |= ((uint32_t)(int)acc[i][j][v] & 1u) << (8 * (v >> 2) + 4 * half + (v & 3));
+			m |= __shfl_xor(m, 32);
+			if(!half) {
+				const size_t r = r0 + 128 * wcol + 32 * j + row32;
+				const size_t wb = (w0 + 128 * wr + 32 * i) / 32;
+				par[((size_t)t * r_pad + r) * wbk + wb] = m;
+			}
+		}
+}
+
+// Recovery words from the parity bits: thread = (output row, 32-word block),
+// blocks varying fastest; the 16 result bit-slices are XORs of the parity masks
+// selected by the recombination table, then transposed to words.
+__global__ __launch_bounds__(256) void k_blm_combine(const uint32_t* __restrict__ par, size_t r_pad, size_t wbk,
+                                                     unsigned n_rows, unsigned r0, uint8_t* __restrict__ out_slots,
+                                                     size_t slot_bytes, size_t w0, int accumulate,
+                                                     const uint16_t* __restrict__ wtab) {
+	const size_t total = (size_t)n_rows * wbk;
+	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const size_t r = g / wbk;
+		const size_t wb = g - r * wbk;
+		uint32_t acc[16];
+#pragma unroll
+		for(int j = 0; j < 16; j++) acc[j] = 0;
+		for(unsigned t = 0; t < kBl; t++) {
+			const uint32_t m = par[((size_t)t * r_pad + r) * wbk + wb];
+			const uint32_t w = wtab[t];
+#pragma unroll
+			for(int j = 0; j < 16; j++) acc[j] ^= m & (0u - ((w >> j) & 1u));
+		}
+		uint32_t* dst = reinterpret_cast<uint32_t*>(out_slots + (r0 + r) * slot_bytes + 2 * (w0 + 32 * wb));
+#pragma unroll
+		for(int k = 0; k < 32; k += 2) {
+			uint32_t lo = 0, hi = 0;
+#pragma unroll
+			for(int j = 0; j < 16; j++) {
+				lo |= ((acc[j] >> k) & 1u) << j;
+				hi |= ((acc[j] >> (k + 1)) & 1u) << j;
+			}
+			const uint32_t o = lo | (hi << 16);
+			dst[k / 2] = accumulate ? (dst[k / 2] ^ o) : o;
+		}
+	}
+}
+
+}  // namespace pph
+
+// ------------------------------------------------------------------ driver
+
+namespace pph {
+namespace {
+
+constexpr size_t kBlmCoefBudget = 16ull << 30;   // coefficient planes per output block
+constexpr size_t kBlmChunkBudget = 16ull << 30;  // word planes + parity bits per word chunk
+
+size_t blm_env(const char* name) {
+	const char* v = getenv(name);
+	return v ? (size_t)strtoull(v, nullptr, 10) : 0;
+}
+
+struct BlmShape {
+	size_t kw_count, r_blk, wc, slot_words;
+	size_t cb_bytes, wb_bytes, p_bytes, total;
+};
+
+BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+	BlmShape s{};
+	s.kw_count = round_up(n_in, 8 * 32) / 32;  // whole groups of four 64-input steps
+	s.slot_words = slot_bytes / 2;               // a multiple of 512
+	const size_t per_row = kBl * s.kw_count * 4;
+	size_t rb = std::max<size_t>(kBlmTile, kBlmCoefBudget / per_row / kBlmTile * kBlmTile);
+	if(size_t cap = blm_env("PPH_BILINEAR_ROWS")) rb = std::min(rb, round_up(cap, kBlmTile));
+	s.r_blk = std::min(rb, round_up(n_out, kBlmTile));
+	const size_t per_word = kBl * s.kw_count * 4 + kBl * s.r_blk / 8;
+	size_t wc = std::max<size_t>(kBlmTile, kBlmChunkBudget / per_word / kBlmTile * kBlmTile);
+	if(size_t cap = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(cap, kBlmTile));
+	wc = std::min(wc, round_up(s.slot_words, kBlmTile));
+	const size_t chunks = (s.slot_words + wc - 1) / wc;
+	s.wc = round_up((s.slot_words + chunks - 1) / chunks, kBlmTile);
+	s.cb_bytes = round_up(kBl * s.kw_count * s.r_blk * 4, 256);
+	s.wb_bytes = round_up(kBl * s.kw_count * s.wc * 4, 256);
+	s.p_bytes = round_up(kBl * s.r_blk * (s.wc / 32) * 4, 256);
+	s.total = s.cb_bytes + s.wb_bytes + s.p_bytes;
+	return s;
+}
+
+}  // namespace
+
+size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+	return blm_shape(slot_bytes, n_in, n_out).total;
+}
+
+int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
+                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s) {
+	if(!n_out) return PPH_OK;
+	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out);
+	if(!ws || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
+	uint32_t* cb = static_cast<uint32_t*>(ws);
+	uint32_t* wb = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes);
+	uint32_t* par = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes + sh.wb_bytes);
+	const size_t cap = (size_t)(tab.cus > 0 ? tab.cus : 256) * 16;
+	auto blocks = [&](size_t threads) { return (unsigned)std::max<size_t>(1, std::min(cap, (threads + 255) / 256)); };
+	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
+		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
+		const size_t r_pad = round_up(rows, kBlmTile);
+		hipLaunchKernelGGL(k_blm_coef, dim3(blocks(sh.kw_count * r_pad)), dim3(256), 0, s, cb, sh.kw_count, r_pad, n_in,
+		                   (unsigned)r0, rows, input_logs, exps, tab.exp);
+		hipError_t e = hipGetLastError();
+		if(e != hipSuccess) return set_error(e);
+		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
+			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 256
+			hipLaunchKernelGGL(k_blm_words, dim3(blocks(sh.kw_count * wc)), dim3(256), 0, s, wb, sh.kw_count, wc, wc,
+			                   (const uint8_t*)in_slots, slot_bytes, n_in, w0);
+			const unsigned w_tiles = (unsigned)(wc / kBlmTile), r_tiles = (unsigned)(r_pad / kBlmTile);
+			if(e == hipSuccess) e = hipGetLastError();
+			if(e == hipSuccess)
+				hipLaunchKernelGGL(k_blm_gemm, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc,
+				                   r_pad, w_tiles, r_tiles);
+			if(e == hipSuccess) e = hipGetLastError();
+			if(e == hipSuccess)
+				hipLaunchKernelGGL(k_blm_combine, dim3(blocks((size_t)rows * (wc / 32))), dim3(256), 0, s, par, r_pad,
+				                   wc / 32, rows, (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0, accumulate ? 1 : 0,
+				                   wtab);
+			if(e == hipSuccess) e = hipGetLastError();
+			if(e != hipSuccess) return set_error(e);
+		}
+	}
+	return PPH_OK;
+}
+
+}  // namespace pph

@@ -82,6 +82,11 @@ int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                      const DeviceTables& tab, hipStream_t s);
+// fp4 MFMA kernel of the bilinear form (pph_bl_mfma.hip).
+size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
+int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
+                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
