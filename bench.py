@@ -50,6 +50,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
+FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 MFMA
 
 # DRAM bytes per mul-add launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by a 1 GiB plain copy in the same runs
@@ -564,14 +565,23 @@ def main():
     value = 2.0 * ma / (ms_per_step / 1e3) / 1e9
 
     if dense:
-        # int8 ops per mul-add: 2 x 256 MACs (16x16 bit-matrix) or 2 x 81 (bilinear formula)
+        # MFMA ops per mul-add: 2 x 256 MACs (16x16 bit-matrix, int8) or 2 x 81 (bilinear
+        # formula: fp4 e2m1 MFMA kernel, or int8 GEMMs with PPH_BILINEAR_GEMM=library)
         per_ma = 512 if args.form == "dense" else 162
+        library = os.environ.get("PPH_BILINEAR_GEMM") == "library"
+        fp4 = args.form == "bilinear" and not library
+        peak = FP4_PEAK_TFLOPS if fp4 else INT8_PEAK_TOPS
         achieved = per_ma * ma / (ker_avg / 1e3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": MEASURED_TRAFFIC.get((args.form, n, S, R)),
-                "kernel": "k_muladd_dense" if args.form == "dense" else
-                "pph_muladd_bilinear (expansion kernels + 81 batched int8 GEMMs + epilogue)",
-                "kernel_ms": round(ker_avg, 3), "algorithmic_int8_ops": per_ma * ma}
+        if args.form == "dense":
+            kname = "k_muladd_dense (int8 MFMA)"
+        elif fp4:
+            kname = "pph_muladd_bilinear: k_blm_gemm (fp4 MFMA) + plane and recombination kernels"
+        else:
+            kname = "pph_muladd_bilinear: 81 batched int8 GEMMs (hipBLASLt) + plane and parity kernels"
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s" if fp4 else "TOP/s",
+                "frac": round(achieved / peak, 4), "traffic": MEASURED_TRAFFIC.get((args.form, n, S, R)),
+                "kernel": kname, "kernel_ms": round(ker_avg, 3), "algorithmic_mfma_ops": per_ma * ma,
+                "operand_type": "fp4 e2m1" if fp4 else "int8"}
         if compare is not None:
             compare["dense_vs_plain"] = round(compare["plain_kernel_ms"] / ker_avg, 3)
             d_rows = out.download(min(R, 16) * slot).reshape(-1, slot)[:, :S]
@@ -605,7 +615,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16 (GF(2^16) words)" + (", int8 0/1 MFMA operands" if dense else ""),
+        "dtype": "u16 (GF(2^16) words)" + ((", %s 0/1 MFMA operands" % roof["operand_type"]) if dense else ""),
         "data": "synthetic (counter-based random bytes, seed %d)" % args.seed,
         "config": {"workload": "%s recovery: %d inputs x %d B slices, %d recovery slices (exp 0..%d)" % (
             args.form, n, S, R, R - 1), "config_index": args.config, "inputs": n, "slice_bytes": S,

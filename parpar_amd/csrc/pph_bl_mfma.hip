@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace pph {
 
@@ -104,13 +105,18 @@ constexpr unsigned kBlmRow = 32;    // LDS bytes per row and 64-input step
 
 // One workgroup: component t, words w0 .. w0+255, outputs r0 .. r0+255.
 // Waves 2x2, each 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
-// kw_count (32-input groups) is a multiple of 8: the loop runs whole groups of
-// four 64-input steps.
+// One step = 64 inputs (two 32-input groups, one MFMA k-step); kw_count is a
+// multiple of 24, so the loop runs whole groups of twelve steps.  LDS holds the
+// bit masks ([group][row], 8 bytes per row and step, three buffers); each wave
+// expands its fragments in registers.  Pipeline per step s, one barrier: MFMAs
+// of step s, the mask reads and expansion of step s + 1, the masks of step
+// s + 2 into LDS and the global fetch of step s + 6.
 __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ wbits, const uint32_t* __restrict__ cb,
                                                   uint32_t* __restrict__ par, size_t kw_count, size_t wc_pad,
                                                   size_t r_pad, unsigned w_tiles, unsigned r_tiles) {
-	__shared__ __align__(16) uint8_t sA[2][kBlmTile * kBlmRow];
-	__shared__ __align__(16) uint8_t sB[2][kBlmTile * kBlmRow];
+	constexpr unsigned kBufs = 3, kDepth = 4;
+	__shared__ __align__(16) uint32_t sA[kBufs][2][kBlmTile];
+	__shared__ __align__(16) uint32_t sB[kBufs][2][kBlmTile];
 	const unsigned tiles = w_tiles * r_tiles;
 	const unsigned t = blockIdx.x / tiles;
 	const unsigned tile = blockIdx.x - t * tiles;
@@ -130,19 +136,24 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 #pragma unroll
 			for(int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
 
+	// LDS layout [group][row]: a half-wave reads 32 consecutive masks
 	auto stage = [&](unsigned buf, const uint32_t* m) {
-		uint4* da = reinterpret_cast<uint4*>(sA[buf] + tid * kBlmRow);
-		uint4* db = reinterpret_cast<uint4*>(sB[buf] + tid * kBlmRow);
-		da[0] = blm_nibbles(m[0]);
-		da[1] = blm_nibbles(m[1]);
-		db[0] = blm_nibbles(m[2]);
-		db[1] = blm_nibbles(m[3]);
+		sA[buf][0][tid] = m[0];
+		sA[buf][1][tid] = m[1];
+		sB[buf][0][tid] = m[2];
+		sB[buf][1][tid] = m[3];
+	};
+	// fragments: this lane's mask of group `half` for its four A and four B rows
+	auto frags = [&](unsigned buf, v8i* fa, v8i* fb) {
+#pragma unroll
+		for(int i = 0; i < 4; i++) {
+			const uint4 qa = blm_nibbles(sA[buf][half][128 * wr + 32 * i + row32]);
+			const uint4 qb = blm_nibbles(sB[buf][half][128 * wcol + 32 * i + row32]);
+			fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
+			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
+		}
 	};
 	const unsigned steps = (unsigned)(kw_count / 2);
-	// masks of step s: A groups 2s, 2s+1 of this thread's word row, B groups of
-	// its output row; fetched kDepth - 1 steps ahead into rotating register sets
-	constexpr unsigned kDepth = 4;
-	uint32_t pf[kDepth][4];
 	// 32-bit element offsets from uniform bases (a plane holds < 2^32 masks)
 	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
 	auto fetch = [&](uint32_t* m, unsigned st) {
@@ -152,37 +163,48 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 		m[2] = B[ib];
 		m[3] = B[ib + ldb];
 	};
+	auto clampst = [&](unsigned st) { return st < steps ? st : steps - 1; };
+	// masks of steps s + 2 .. s + 5 live in pf[(s + 2) % 4] .. (steps past the
+	// end repeat the last one: in bounds, staged but never read)
+	uint32_t pf[kDepth][4];
+	{
+		uint32_t m0[4], m1[4];
+		fetch(m0, 0);
+		fetch(m1, clampst(1));
+		stage(0, m0);
+		stage(1, m1);
+	}
 #pragma unroll
-	for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], k);  // steps >= kDepth
-	stage(0, pf[0]);
+	for(unsigned k = 0; k < kDepth; k++) fetch(pf[(2 + k) % kDepth], clampst(2 + k));
 	__syncthreads();
-	for(unsigned s0 = 0; s0 < steps; s0 += kDepth) {
+	v8i fa[2][4], fb[2][4];
+	frags(0, fa[0], fb[0]);
+	for(unsigned s0 = 0; s0 < steps; s0 += 12) {
+		// twelve steps per pass: a common multiple of the three LDS buffers, the two
+		// fragment sets and the four prefetch sets, so every index is static
 #pragma unroll
-		for(unsigned k = 0; k < kDepth; k++) {
-			const unsigned s = s0 + k;  // steps is a multiple of kDepth
-			const unsigned buf = s & 1;
-			v8i fa[4], fb[4];
-#pragma unroll
-			for(int i = 0; i < 4; i++) {
-				const uint4 qa =
-				    *reinterpret_cast<const uint4*>(sA[buf] + (128 * wr + 32 * i + row32) * kBlmRow + 16 * half);
-				const uint4 qb =
-				    *reinterpret_cast<const uint4*>(sB[buf] + (128 * wcol + 32 * i + row32) * kBlmRow + 16 * half);
-				fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
-				fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
-			}
+		for(unsigned k = 0; k < 12; k++) {
+			const unsigned s = s0 + k;  // steps is a multiple of 12
+			const unsigned cur = k & 1, nxt = cur ^ 1;
+			frags((k + 1) % kBufs, fa[nxt], fb[nxt]);
 #pragma unroll
 			for(int i = 0; i < 4; i++)
 #pragma unroll
 				for(int j = 0; j < 4; j++)
-					acc[i][j] =
-					    __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 4, 4, 0, 127, 0, 127);
-			// step s + 1 into the other buffer (its masks arrived kDepth - 1 steps ago),
-			// then the fetch of step s + kDepth into the set step s used.  Both run
-			// unconditionally so that the load counters stay exact: past the end the
-			// fetch repeats the last step (in bounds) and the staged copy is never read.
-			stage(buf ^ 1, pf[(k + 1) % kDepth]);
-			fetch(pf[k], s + kDepth < steps ? s + kDepth : steps - 1);
+					acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[cur][i], fb[cur][j], acc[i][j], 4, 4,
+					                                                            0, 127, 0, 127);
+			stage((k + 2) % kBufs, pf[(k + 2) % kDepth]);
+			fetch(pf[(k + 2) % kDepth], clampst(s + 2 + kDepth));
+			// order: mask reads first, then MFMAs interleaved with the expansion,
+			// LDS writes and global loads
+			__builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS reads
+#pragma unroll
+			for(int g = 0; g < 16; g++) {
+				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+				__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // 4 VALU
+				if(g % 4 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+				if(g % 4 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+			}
 			__syncthreads();
 		}
 	}
@@ -262,7 +284,7 @@ struct BlmShape {
 
 BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 	BlmShape s{};
-	s.kw_count = round_up(n_in, 8 * 32) / 32;  // whole groups of four 64-input steps
+	s.kw_count = round_up(n_in, 24 * 32) / 32;  // whole groups of twelve 64-input steps
 	s.slot_words = slot_bytes / 2;               // a multiple of 512
 	const size_t per_row = kBl * s.kw_count * 4;
 	size_t rb = std::max<size_t>(kBlmTile, kBlmCoefBudget / per_row / kBlmTile * kBlmTile);
