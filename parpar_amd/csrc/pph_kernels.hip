@@ -247,6 +247,31 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep(uint8_t* slots, const uin
 	}
 }
 
+// Data-form preparation (no log table): 256-thread workgroups, several per CU,
+// persistent over slots — the form the dense kernels read.
+template <unsigned CK>
+__global__ __launch_bounds__(kCkThreads) void k_prep_data(uint8_t* slots, const uint8_t* src_slots, size_t slot_bytes,
+                                                          unsigned count, const uint32_t* __restrict__ lens,
+                                                          const uint32_t* __restrict__ slot_index, size_t slice_size,
+                                                          size_t ck_off, const uint16_t* __restrict__ lg,
+                                                          const uint16_t* __restrict__ ex) {
+	const size_t nrows = slot_bytes / CK;
+	const size_t nb = (slice_size + CK - 1) / CK;
+	for(unsigned k = blockIdx.x; k < count; k += gridDim.x) {
+		const size_t sidx = slot_index ? slot_index[k] : k;
+		uint8_t* slot = slots + sidx * slot_bytes;
+		const uint8_t* src = src_slots ? src_slots + sidx * slot_bytes : slot;
+		size_t len = lens ? lens[k] : slice_size;
+		if(len > slice_size) len = slice_size;
+		uint32_t h[CkGeom<CK, kCkThreads>::DW];
+		size_t run_end;
+		ck_rows<CK, kCkThreads, 1>(src, slot, nrows, nb, len, h, nullptr, run_end);
+		__threadfence();  // row rewrites complete before the checksum block is stored over them
+		uint32_t lane = ck_reduce<CK, kCkThreads>(h, nb, run_end, lg, ex);
+		if(threadIdx.x < CK / 2) *reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = (uint16_t)lane;
+	}
+}
+
 template <unsigned CK>
 __global__ __launch_bounds__(kCkThreads) void k_verify(const uint8_t* __restrict__ slots, size_t slot_bytes,
                                                        size_t slice_size, size_t ck_off, int* __restrict__ valid,
@@ -276,9 +301,21 @@ hipError_t launch_prep(uint8_t* slots, const uint8_t* src, size_t slot_bytes, un
 	if(!count) return hipSuccess;
 	if(slot_bytes % kSlotAlign || ck_off + ck > slot_bytes || slice_size > ck_off || !valid_ck(ck))
 		return hipErrorInvalidValue;
-	const unsigned blocks = count < (unsigned)(t.cus > 0 ? t.cus : 256) ? count : (unsigned)(t.cus > 0 ? t.cus : 256);
+	const unsigned cus = (unsigned)(t.cus > 0 ? t.cus : 256);
+	if(!to_log) {
+		const unsigned blocks = count < cus * 8 ? count : cus * 8;
+		dim3 grid(blocks), block(kCkThreads);
+		switch(ck) {
+			case 8: hipLaunchKernelGGL(k_prep_data<8>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, t.log, t.exp); break;
+			case 16: hipLaunchKernelGGL(k_prep_data<16>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, t.log, t.exp); break;
+			case 32: hipLaunchKernelGGL(k_prep_data<32>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, t.log, t.exp); break;
+			default: hipLaunchKernelGGL(k_prep_data<64>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, t.log, t.exp); break;
+		}
+		return hipGetLastError();
+	}
+	const unsigned blocks = count < cus ? count : cus;
 	dim3 grid(blocks), block(kPrepThreads);
-	const int tl = to_log ? 1 : 0;
+	const int tl = 1;
 	switch(ck) {
 		case 8: hipLaunchKernelGGL(k_prep<8>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
 		case 16: hipLaunchKernelGGL(k_prep<16>, grid, block, 0, s, slots, src, slot_bytes, count, lens, slot_index, slice_size, ck_off, tl, t.log, t.exp); break;
