@@ -123,9 +123,32 @@ def main_serving(args):
     name = ctypes.create_string_buffer(256)
     cus = ctypes.c_int()
     L.pph_device_info(0, name, 256, ctypes.byref(cus))
-    srv = SV.Server(budget, max_compute=args.max_compute, bilinear_workspace=bl_ws)
-    log("arena %.1f GiB, %d compute streams, bilinear workspace %d GiB" % (
-        budget / 2**30, args.max_compute, args.bilinear_workspace))
+    pool = None
+    if args.host_data:
+        # host-resident files: every file is a window of one host pool (random bytes from the
+        # device generator), read through the pinned staging path and hashed on host threads
+        pool_bytes = max(args.pool_gib << 30, max(a[1] for a in arrivals))
+        pool = np.empty(pool_bytes, np.uint8)
+        chunk = 1 << 30
+        buf = parpar_amd.proc.DeviceBuffer(chunk)
+        for c in range(0, pool_bytes, chunk):
+            n = min(chunk, pool_bytes - c)
+            chk(L.pph_fill_random(buf.ptr, n, 1, n, args.seed, 0xD0D0, c // chunk, None))
+            chk(L.pph_synchronize())
+            chk(L.pph_memcpy_d2h(parpar_amd.proc._ptr(pool[c:c + n]), buf.ptr, n))
+        buf.close()
+        log("host pool %.1f GiB" % (pool_bytes / 2**30))
+
+    def pool_off(k, T):
+        return (k * 0x9E3779B1 * 4096 + 12345) % (len(pool) - T + 1)
+
+    srv = SV.Server(budget, max_compute=args.max_compute, bilinear_workspace=bl_ws,
+                    md5_host_threads=args.md5_host_threads if args.host_data else 0,
+                    upload_threads=args.upload_threads if args.host_data else 0)
+    log("arena %.1f GiB, %d compute streams, bilinear workspace %d GiB%s" % (
+        budget / 2**30, args.max_compute, args.bilinear_workspace,
+        ", host data: %d MD5 threads, %d upload threads" % (args.md5_host_threads, args.upload_threads)
+        if args.host_data else ""))
     # warm-up job (tables, first launches), not counted
     srv.run_all([SV.FileJob(8 << 20, 1 << 20, 4, seed=args.seed, job_id=1 << 30)])
 
@@ -135,7 +158,9 @@ def main_serving(args):
     rec_sample = sorted(set(small[:6] + by_size[len(by_size) // 2:len(by_size) // 2 + 3] + by_size[-2:]))
     keep = set(small) | set(rec_sample)
 
-    jobs = [SV.FileJob(T, S, R, seed=args.seed, job_id=k) for k, (_, T, S, R) in enumerate(arrivals)]
+    jobs = [SV.FileJob(T, S, R, seed=args.seed, job_id=k,
+                       data=None if pool is None else pool[pool_off(k, T):pool_off(k, T) + T])
+            for k, (_, T, S, R) in enumerate(arrivals)]
     t0 = time.perf_counter()
 
     def submitter():
@@ -184,8 +209,12 @@ def main_serving(args):
         t_cpu, cma = 0.0, 0
         for k in small:
             job = kept[k]
-            data = np.concatenate([O.prng_bytes(args.seed, k, i, min(job.slice_size, job.file_size - i * job.slice_size))
-                                   for i in range(job.n_in)])
+            if pool is not None:
+                data = job.data
+            else:
+                data = np.concatenate([O.prng_bytes(args.seed, k, i,
+                                                    min(job.slice_size, job.file_size - i * job.slice_size))
+                                       for i in range(job.n_in)])
             ok = ok and bytes(job.file_md5).hex() == hashlib.md5(data.tobytes()).hexdigest()
             pad = np.zeros(job.n_in * job.slice_size, np.uint8)
             pad[:job.file_size] = data
@@ -205,7 +234,9 @@ def main_serving(args):
             lens[-1] = max(0, min(win, last - off))
             inp = np.zeros((n, win), np.uint8)
             for i in range(n):
-                if lens[i]:
+                if lens[i] and pool is not None:
+                    inp[i, :lens[i]] = job.data[i * S + off:i * S + off + lens[i]]
+                elif lens[i]:
                     inp[i, :lens[i]] = O.prng_bytes(args.seed, k, i, lens[i], offset=off)
             tc = time.perf_counter()
             ref = O.recover(inp, lens, win, np.arange(job.n_out, dtype=np.uint16), threads=threads)
@@ -225,7 +256,11 @@ def main_serving(args):
         "value": round(achieved, 2), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(makespan * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u16 (GF(2^16) words); MD5 / CRC-32 bytes",
-        "data": "synthetic (counter-based random bytes generated on the device, seed %d)" % args.seed,
+        "data": ("synthetic (counter-based random bytes generated on the device, seed %d)" % args.seed
+                 if pool is None else
+                 "synthetic, host-resident: each file a window of a %.0f GiB host pool of random bytes, "
+                 "copied through pinned staging by %d threads; file MD5 on %d host threads" % (
+                     len(pool) / 2**30, args.upload_threads, args.md5_host_threads)),
         "config": {"workload": "serving: Poisson 20 files/s for %.0f s, file 1 MiB..8 GiB, slice 256 KiB..16 MiB, "
                                "recovery 1..30%% of slices" % args.window, "config_index": 4,
                    "files": len(arrivals), "file_bytes": total_bytes, "mul_adds": total_ma,
@@ -241,7 +276,7 @@ def main_serving(args):
         "achieved": {"bytes_per_s": round(total_bytes / makespan / 1e9, 3), "ma_per_s": total_ma / makespan,
                      "makespan_s": round(makespan, 2)},
         "engine": {"peak_jobs": st["peak_jobs"], "peak_arena_bytes": st["peak_bytes"], "md5_rounds": st["md5_rounds"],
-                   "bilinear_jobs": st["bilinear_jobs"]},
+                   "bilinear_jobs": st["bilinear_jobs"], "host_md5_jobs": st["host_md5_jobs"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole serving pipeline (mul-add-equivalent bytes over the makespan)"},
@@ -384,6 +419,11 @@ def parse():
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
     ap.add_argument("--bilinear-workspace", type=int, default=40,
                     help="serving: GiB of device workspace for the bilinear form (0: plain only)")
+    ap.add_argument("--host-data", action="store_true",
+                    help="serving: files in host memory (pinned staging uploads, file MD5 on host threads)")
+    ap.add_argument("--pool-gib", type=int, default=16, help="serving --host-data: host pool size")
+    ap.add_argument("--md5-host-threads", type=int, default=12)
+    ap.add_argument("--upload-threads", type=int, default=4)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")

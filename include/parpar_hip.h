@@ -289,6 +289,12 @@ typedef struct pph_serve_config {
 	                               budget (0: plain form only).  Jobs whose predicted bilinear time is
 	                               lower and whose workspace fits use it, one job's GEMMs at a time. */
 	const struct pph_tuning* tuning;  /* rates for that choice; NULL runs pph_autotune at create */
+	unsigned md5_host_threads;  /* > 0: this many host threads compute the file MD5 of jobs whose bytes are in
+	                               host memory, each file from submit on, beside the device work (one core
+	                               runs an MD5 chain several times faster than one wavefront); 0: every
+	                               file MD5 on the device */
+	unsigned upload_threads;    /* threads copying host-sourced inputs to the device, each with its own two
+	                               pinned staging chunks (0 = 1, at most 16) */
 } pph_serve_config;
 
 /* One file.  Its slices (slice_size bytes, the last one partial) are PAR2
@@ -329,6 +335,7 @@ typedef struct pph_serve_result {
 typedef struct pph_serve_stats {
 	uint64_t submitted, completed, md5_rounds;
 	uint64_t bilinear_jobs;     /* completed jobs that used the bilinear form */
+	uint64_t host_md5_jobs;     /* completed jobs whose file MD5 ran on the host */
 	unsigned peak_jobs;         /* most jobs admitted at once */
 	uint64_t peak_bytes;        /* most arena bytes in use at once */
 	uint64_t arena_bytes;

@@ -10,21 +10,24 @@
 // preparation, library GEMMs in an engine-owned workspace; the bilinear
 // phases of different jobs are ordered through one event so that they take
 // the workspace one at a time).
-// Inputs come from the caller's host buffer through two pinned staging chunks
-// (double-buffered hipMemcpyAsync, on an upload thread), or from the
-// counter-based generator on the device.
+// Inputs come from the caller's host buffer through pinned staging chunks
+// (double-buffered hipMemcpyAsync; one or more upload threads, each with its
+// own pair of chunks), or from the counter-based generator on the device.
 //
 // The whole-file MD5 is one serial chain per file.  Running it as a kernel per
 // job would hold a hardware queue for seconds on large files, so every file
 // whose inputs are on the device joins a shared hash round instead: one
 // kernel on the hash stream, one wavefront per file, each advancing its chain
 // by at most `md5_quantum` bytes.  Rounds repeat while chains are unfinished;
-// files join and leave at round boundaries.
+// files join and leave at round boundaries.  Optionally, files whose bytes
+// are in host memory are hashed by a pool of host threads instead, from
+// submit on (a core runs the chain several times faster than a wavefront).
 //
 // A job's memory is two arena ranges: the inputs and hash results (held until
 // its MD5 chain ends) and the log-domain inputs and recovery slices (released
 // as soon as the recovery and its checks are copied back).
 #include "pph_internal.h"
+#include "pph_md5_host.h"
 #include "../../include/parpar_hip.h"
 
 #include <algorithm>
@@ -91,6 +94,7 @@ public:
 constexpr unsigned kMaxBypass = 16;   // admissions that may pass a waiting head job
 constexpr unsigned kLookBehind = 8;   // queued jobs considered behind the head
 constexpr unsigned kMaxSpans = 1024;  // files per hash round
+constexpr unsigned kMaxUploaders = 16;
 
 struct Job {
 	pph_serve_job spec;
@@ -115,13 +119,22 @@ struct Job {
 	bool computed = false;   // compute results copied back, B released (guarded by the engine mutex)
 	bool hashing = false;    // in the MD5 rounds
 	uint64_t md5_pos = 0;    // bytes of the chain dispatched
-	bool md5_done = false;
+	bool md5_done = false;   // for host_md5 jobs: set by the hashing thread, guarded by the engine mutex
+	bool host_md5 = false;   // file MD5 on a host thread (fixed at submit)
+	uint8_t host_digest[16] = {};
 	int status = PPH_OK;
 };
 
 struct Round {
 	hipEvent_t ev = nullptr;
 	std::vector<Job*> finals;  // jobs whose last span is in this round
+};
+
+// Two pinned chunks of one upload thread: the host fills one while the
+// other's copy is in flight.
+struct Stager {
+	uint8_t* buf[2] = {nullptr, nullptr};
+	hipEvent_t ev[2] = {nullptr, nullptr};  // last copy out of each chunk
 };
 
 }  // namespace
@@ -144,8 +157,7 @@ struct pph_serve {
 	hipEvent_t round_ev[2] = {nullptr, nullptr};
 	std::deque<Round> rounds;  // in flight, oldest first
 	unsigned next_round_buf = 0;
-	uint8_t* pinned[2] = {nullptr, nullptr};
-	hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+	std::vector<Stager> stagers;  // one per upload thread
 	void* bl_ws = nullptr;               // bilinear workspace
 	hipEvent_t bl_tail = nullptr;        // end of the last enqueued bilinear phase
 	bool bl_any = false;
@@ -158,12 +170,14 @@ struct pph_serve {
 	std::vector<Job*> active;  // admitted, not finished
 	std::deque<Job*> uploads;
 	std::deque<Job*> deliveries;
+	std::deque<Job*> host_md5;  // submitted, file MD5 not yet started on a host thread
 	std::vector<pph_serve_result> done;
 	pph_serve_stats stats{};
 	uint64_t next_ticket = 1;
 	unsigned outstanding = 0;
 	bool stop = false, sched_done = false;
-	std::thread sched, uploader, deliverer;
+	std::thread sched, deliverer;
+	std::vector<std::thread> uploaders, md5_threads;
 };
 
 namespace {
@@ -291,7 +305,7 @@ bool admit(pph_serve* s, Job* j) {
 
 // Host-sourced inputs: chunked copies through two pinned buffers (the host
 // fills one while the other is in flight), then the compute chain.
-void upload_job(pph_serve* s, Job* j) {
+void upload_job(pph_serve* s, Job* j, Stager& st) {
 	hipStream_t a = s->streams[j->stream];
 	const uint8_t* src = (const uint8_t*)j->spec.host_data;
 	const size_t S = j->spec.slice_size, chunk = s->cfg.staging_bytes;
@@ -301,11 +315,11 @@ void upload_job(pph_serve* s, Job* j) {
 		const size_t len = k + 1 < j->n_in ? S : j->last_len;
 		for(size_t done = 0; done < len && e == hipSuccess;) {
 			const size_t n = std::min(chunk, len - done);
-			e = hipEventSynchronize(s->pinned_ev[which]);  // the last copy out of this buffer has finished
+			e = hipEventSynchronize(st.ev[which]);  // the last copy out of this chunk has finished
 			if(e != hipSuccess) break;
-			memcpy(s->pinned[which], src + k * S + done, n);
-			e = hipMemcpyAsync(j->data + k * j->slot + done, s->pinned[which], n, hipMemcpyHostToDevice, a);
-			if(e == hipSuccess) e = hipEventRecord(s->pinned_ev[which], a);
+			memcpy(st.buf[which], src + k * S + done, n);
+			e = hipMemcpyAsync(j->data + k * j->slot + done, st.buf[which], n, hipMemcpyHostToDevice, a);
+			if(e == hipSuccess) e = hipEventRecord(st.ev[which], a);
 			which ^= 1u;
 			done += n;
 		}
@@ -352,8 +366,12 @@ void finish_job(pph_serve* s, Job* j) {
 	}
 	for(Round& r : s->rounds) r.finals.erase(std::remove(r.finals.begin(), r.finals.end(), j), r.finals.end());
 	if(j->status == PPH_OK && sp.file_md5) {
-		hipError_t e = hipMemcpy(sp.file_md5, j->digest, 16, hipMemcpyDeviceToHost);
-		if(e != hipSuccess) j->status = set_error(e);
+		if(j->host_md5) {
+			memcpy(sp.file_md5, j->host_digest, 16);
+		} else {
+			hipError_t e = hipMemcpy(sp.file_md5, j->digest, 16, hipMemcpyDeviceToHost);
+			if(e != hipSuccess) j->status = set_error(e);
+		}
 	}
 	const double t = now_s();
 	pph_serve_result r{};
@@ -374,6 +392,7 @@ void finish_job(pph_serve* s, Job* j) {
 	s->done.push_back(r);
 	s->stats.completed++;
 	if(j->form == PPH_FORM_BILINEAR) s->stats.bilinear_jobs++;
+	if(j->host_md5) s->stats.host_md5_jobs++;
 	if(r.status == PPH_OK) {
 		s->stats.mul_adds += r.mul_adds;
 		s->stats.bytes += r.bytes;
@@ -510,7 +529,7 @@ void scheduler(pph_serve* s) {
 				deliver(s, j);
 				progress = true;
 			}
-			if(j->hashing && !j->md5_done) hashing.push_back(j);
+			if(j->hashing && !j->host_md5 && !j->md5_done) hashing.push_back(j);
 		}
 		// finished rounds, oldest first
 		while(!s->rounds.empty() && hipEventQuery(s->rounds.front().ev) == hipSuccess) {
@@ -521,12 +540,14 @@ void scheduler(pph_serve* s) {
 		// keep two rounds queued on the hash stream
 		while(s->rounds.size() < 2 && launch_round(s, hashing)) progress = true;
 		for(Job* j : act) {
-			bool computed;
+			bool computed, md5_done;
 			{
 				std::lock_guard<std::mutex> lk(s->mu);
 				computed = j->computed;
+				md5_done = j->md5_done;
 			}
-			if(computed && (j->md5_done || j->status != PPH_OK)) {
+			// a host thread may still read a failed job's bytes: it ends only after its hash
+			if(computed && (md5_done || (j->status != PPH_OK && !j->host_md5))) {
 				finish_job(s, j);
 				progress = true;
 			}
@@ -553,7 +574,7 @@ void deliverer(pph_serve* s) {
 	}
 }
 
-void uploader(pph_serve* s) {
+void uploader(pph_serve* s, unsigned k) {
 	(void)hipSetDevice(s->device);
 	for(;;) {
 		Job* j = nullptr;
@@ -564,7 +585,27 @@ void uploader(pph_serve* s) {
 			j = s->uploads.front();
 			s->uploads.pop_front();
 		}
-		upload_job(s, j);
+		upload_job(s, j, s->stagers[k]);
+	}
+}
+
+// File MD5 of host-sourced jobs, oldest submit first.  No device calls.
+void md5_worker(pph_serve* s) {
+	for(;;) {
+		Job* j = nullptr;
+		{
+			std::unique_lock<std::mutex> lk(s->mu);
+			s->cv.wait(lk, [&] { return s->sched_done || !s->host_md5.empty(); });
+			if(s->host_md5.empty()) break;
+			j = s->host_md5.front();
+			s->host_md5.pop_front();
+		}
+		HostMd5 h;
+		h.update((const uint8_t*)j->spec.host_data, (size_t)j->spec.file_size);
+		h.final(j->host_digest);
+		std::lock_guard<std::mutex> lk(s->mu);
+		j->md5_done = true;
+		s->cv.notify_all();
 	}
 }
 
@@ -583,7 +624,10 @@ extern "C" void pph_serve_destroy(pph_serve* s) {
 		s->sched_done = true;
 		s->cv.notify_all();
 	}
-	if(s->uploader.joinable()) s->uploader.join();
+	for(auto& t : s->uploaders)
+		if(t.joinable()) t.join();
+	for(auto& t : s->md5_threads)
+		if(t.joinable()) t.join();
 	if(s->deliverer.joinable()) s->deliverer.join();
 	(void)hipSetDevice(s->device);
 	for(auto st : s->streams)
@@ -595,9 +639,12 @@ extern "C" void pph_serve_destroy(pph_serve* s) {
 		(void)hipStreamSynchronize(s->hash_stream);
 		(void)hipStreamDestroy(s->hash_stream);
 	}
+	for(Stager& st : s->stagers)
+		for(int k = 0; k < 2; k++) {
+			if(st.ev[k]) (void)hipEventDestroy(st.ev[k]);
+			if(st.buf[k]) (void)hipHostFree(st.buf[k]);
+		}
 	for(int k = 0; k < 2; k++) {
-		if(s->pinned_ev[k]) (void)hipEventDestroy(s->pinned_ev[k]);
-		if(s->pinned[k]) (void)hipHostFree(s->pinned[k]);
 		if(s->round_ev[k]) (void)hipEventDestroy(s->round_ev[k]);
 		if(s->h_spans[k]) (void)hipHostFree(s->h_spans[k]);
 		if(s->d_spans[k]) (void)hipFree(s->d_spans[k]);
@@ -614,7 +661,10 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	pph_serve_config c = *cfg;
 	if(!c.md5_quantum) c.md5_quantum = 8u << 20;
 	if(!c.staging_bytes) c.staging_bytes = 64u << 20;
-	if(!c.hbm_budget || !c.max_compute || c.max_compute > 64 || c.md5_quantum % 64) return PPH_E_INVALID;
+	if(!c.upload_threads) c.upload_threads = 1;
+	if(!c.hbm_budget || !c.max_compute || c.max_compute > 64 || c.md5_quantum % 64 ||
+	   c.upload_threads > kMaxUploaders || c.md5_host_threads > 256)
+		return PPH_E_INVALID;
 	int n = 0;
 	if(hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PPH_E_UNSUPPORTED;
 	if(device < 0 || device >= n) return PPH_E_INVALID;
@@ -641,11 +691,15 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	for(auto& st : s->streams)
 		if(e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
 	if(e == hipSuccess) e = hipStreamCreateWithFlags(&s->hash_stream, hipStreamNonBlocking);
+	s->stagers.resize(c.upload_threads);
+	for(Stager& st : s->stagers)
+		for(int k = 0; k < 2 && e == hipSuccess; k++) {
+			e = hipHostMalloc((void**)&st.buf[k], c.staging_bytes, hipHostMallocDefault);
+			if(e == hipSuccess) e = hipEventCreateWithFlags(&st.ev[k], hipEventDisableTiming);
+			if(e == hipSuccess) e = hipEventRecord(st.ev[k], s->hash_stream);  // "chunk free"
+		}
 	for(int k = 0; k < 2 && e == hipSuccess; k++) {
-		e = hipHostMalloc((void**)&s->pinned[k], c.staging_bytes, hipHostMallocDefault);
-		if(e == hipSuccess) e = hipEventCreateWithFlags(&s->pinned_ev[k], hipEventDisableTiming);
-		if(e == hipSuccess) e = hipEventRecord(s->pinned_ev[k], s->hash_stream);  // "buffer free"
-		if(e == hipSuccess) e = hipHostMalloc((void**)&s->h_spans[k], kMaxSpans * sizeof(Md5Span), hipHostMallocDefault);
+		e = hipHostMalloc((void**)&s->h_spans[k], kMaxSpans * sizeof(Md5Span), hipHostMallocDefault);
 		if(e == hipSuccess) e = hipMalloc((void**)&s->d_spans[k], kMaxSpans * sizeof(Md5Span));
 		if(e == hipSuccess) e = hipEventCreateWithFlags(&s->round_ev[k], hipEventDisableTiming);
 	}
@@ -671,7 +725,8 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	}
 	s->stats.arena_bytes = c.hbm_budget;
 	s->sched = std::thread(scheduler, s);
-	s->uploader = std::thread(uploader, s);
+	for(unsigned k = 0; k < c.upload_threads; k++) s->uploaders.emplace_back(uploader, s, k);
+	for(unsigned k = 0; k < c.md5_host_threads; k++) s->md5_threads.emplace_back(md5_worker, s);
 	s->deliverer = std::thread(deliverer, s);
 	*out = s;
 	return PPH_OK;
@@ -712,6 +767,7 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 		delete j;
 		return PPH_E_NOMEM;  // could never be admitted
 	}
+	j->host_md5 = s->cfg.md5_host_threads && job->host_data;
 	j->t_arrival = now_s();
 	std::lock_guard<std::mutex> lk(s->mu);
 	if(s->stop) {
@@ -721,6 +777,7 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 	j->ticket = s->next_ticket++;
 	if(ticket) *ticket = j->ticket;
 	s->pending.push_back(j);
+	if(j->host_md5) s->host_md5.push_back(j);
 	s->outstanding++;
 	s->stats.submitted++;
 	s->cv.notify_all();

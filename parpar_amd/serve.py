@@ -17,7 +17,7 @@ from .proc import _ptr
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_compute", ctypes.c_uint), ("md5_quantum", ctypes.c_size_t),
                 ("staging_bytes", ctypes.c_size_t), ("bilinear_workspace", ctypes.c_size_t),
-                ("tuning", ctypes.c_void_p)]
+                ("tuning", ctypes.c_void_p), ("md5_host_threads", ctypes.c_uint), ("upload_threads", ctypes.c_uint)]
 
 
 class ServeJobDesc(ctypes.Structure):
@@ -39,7 +39,8 @@ class ServeResult(ctypes.Structure):
 
 class ServeStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("md5_rounds", ctypes.c_uint64),
-                ("bilinear_jobs", ctypes.c_uint64), ("peak_jobs", ctypes.c_uint), ("peak_bytes", ctypes.c_uint64),
+                ("bilinear_jobs", ctypes.c_uint64), ("host_md5_jobs", ctypes.c_uint64), ("peak_jobs", ctypes.c_uint),
+                ("peak_bytes", ctypes.c_uint64),
                 ("arena_bytes", ctypes.c_uint64),
                 ("mul_adds", ctypes.c_double), ("bytes", ctypes.c_double)]
 
@@ -90,14 +91,17 @@ class Server:
     them."""
 
     def __init__(self, hbm_budget, max_compute=8, md5_quantum=0, staging_bytes=0, device=0, bilinear_workspace=0,
-                 tuning=None):
+                 tuning=None, md5_host_threads=0, upload_threads=0):
         """bilinear_workspace > 0 lets large jobs use the bilinear form; tuning
         (parpar_amd.batch.PPHTuning) gives the rates that choose it, None
-        measures them at create."""
+        measures them at create.  md5_host_threads > 0 hashes files given as
+        host bytes on that many host threads; upload_threads copy host bytes
+        to the device (0 = 1)."""
         self._tuning = tuning  # kept alive while the engine reads it
         self.cfg = ServeConfig(int(hbm_budget), int(max_compute), int(md5_quantum), int(staging_bytes),
                                int(bilinear_workspace),
-                               ctypes.cast(ctypes.pointer(tuning), ctypes.c_void_p) if tuning is not None else None)
+                               ctypes.cast(ctypes.pointer(tuning), ctypes.c_void_p) if tuning is not None else None,
+                               int(md5_host_threads), int(upload_threads))
         h = ctypes.c_void_p()
         check(lib().pph_serve_create(device, ctypes.byref(self.cfg), ctypes.byref(h)), "pph_serve_create")
         self.h = h

@@ -78,6 +78,23 @@ def test_no_device_reports_unsupported():
         parpar_amd.require_device()
 
 
+def test_serve_config_validation():
+    """pph_serve_create checks its settings before touching a device."""
+    from parpar_amd import serve as SV
+    L = parpar_amd.lib()
+    h = ctypes.c_void_p()
+    base = dict(hbm_budget=1 << 20, max_compute=2, md5_quantum=0, staging_bytes=0, bilinear_workspace=0, tuning=None,
+                md5_host_threads=0, upload_threads=0)
+    for bad in (dict(upload_threads=17), dict(md5_host_threads=257), dict(max_compute=0), dict(max_compute=65),
+                dict(md5_quantum=100), dict(hbm_budget=0)):
+        cfg = SV.ServeConfig(**dict(base, **bad))
+        assert L.pph_serve_create(0, ctypes.byref(cfg), ctypes.byref(h)) == parpar_amd.PPH_E_INVALID, bad
+    assert L.pph_serve_create(0, None, ctypes.byref(h)) == parpar_amd.PPH_E_INVALID
+    if parpar_amd.device_count() == 0:
+        cfg = SV.ServeConfig(**dict(base, md5_host_threads=4, upload_threads=16))
+        assert L.pph_serve_create(0, ctypes.byref(cfg), ctypes.byref(h)) == parpar_amd.PPH_E_UNSUPPORTED
+
+
 def test_controller_allocation_rules():
     # PAR2Proc::checkBackendAllocation (gf16/controller.cpp:42-84): even offsets,
     # full cover, odd size only for the last range

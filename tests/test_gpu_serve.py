@@ -145,6 +145,39 @@ def test_serve_bilinear_form():
         assert (getattr(plain, name) == getattr(bl, name)).all(), name
 
 
+def test_serve_host_md5_and_uploaders():
+    """Host-sourced files hashed on host threads and copied by several upload
+    threads (small staging chunks, many copies per slice) give the same bytes
+    as device hashing, alone and mixed."""
+    jobs = mixed_jobs()
+    n_host = sum(1 for j in jobs if j.data is not None)
+    say("serve host md5", n_host)
+    with SV.Server(hbm_budget=256 * MiB, max_compute=4, staging_bytes=12288, md5_host_threads=3,
+                   upload_threads=3) as srv:
+        srv.run_all(jobs)
+        st = srv.stats()
+    say(" stats", st)
+    assert st["host_md5_jobs"] == n_host
+    for job in jobs:
+        check_job(job)
+
+    def target():
+        fs = 11 * 16384 + 5
+        return SV.FileJob(fs, 16384, 4, data=O.prng_bytes(9, 3, 0, fs), first_input=40)
+
+    dev = target()
+    with SV.Server(hbm_budget=64 * MiB, max_compute=1, md5_quantum=4096) as srv:
+        srv.run_all([dev])
+        assert srv.stats()["host_md5_jobs"] == 0
+    host = target()
+    with SV.Server(hbm_budget=24 * MiB, max_compute=3, md5_host_threads=1, upload_threads=2,
+                   staging_bytes=5000) as srv:
+        srv.run_all(mixed_jobs()[:5] + [host] + mixed_jobs()[5:])
+    for name in ("recovery", "valid", "slice_md5", "slice_crc", "file_md5"):
+        assert (getattr(dev, name) == getattr(host, name)).all(), name
+    check_job(host)
+
+
 @pytest.mark.parametrize("quantum", [64, 4096, 8 * MiB])
 def test_serve_md5_rounds(quantum):
     """Files of many sizes hash in shared rounds of `quantum` bytes per chain."""
