@@ -8,9 +8,10 @@
 //   Wb[t][kw][w]  bit of E(x_{32kw + k}[w])_t     (input words w)
 // with element k of a 32-input group at bit 4(k % 8) + k / 8, so that nibble
 // dword q of the fp4 expansion is ((m >> q) & 0x11111111) << 1.  The GEMM
-// kernel (one workgroup per 256 words x 256 outputs x component t) expands
-// them into LDS (double-buffered, one 64-input step per iteration), runs 16
-// MFMAs per wave per step and writes the parity of each sum as a bit:
+// kernel (one workgroup per 256 words x 256 outputs, or 512 words x 128
+// outputs, and component t) stages them in LDS (one 64-input step per
+// iteration), runs 16 MFMAs per wave per step and writes the parity of each
+// sum as a bit:
 //   P[t][r][w / 32]  bit w % 32 = parity of sum_i E(c_ri)_t E(x_i[w])_t
 // k_blm_combine recombines the 81 parity bits of each word into the 16-bit
 // recovery word (bit-sliced over 32 words) and stores or XORs it.
@@ -100,32 +101,38 @@ __global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits,
 	}
 }
 
-constexpr unsigned kBlmTile = 256;  // words and outputs per workgroup
-constexpr unsigned kBlmRow = 32;    // LDS bytes per row and 64-input step
+constexpr unsigned kBlmWordTile = 512;  // word chunks are multiples of this
+constexpr unsigned kBlmKStep = 256;     // inputs are padded to multiples of this
 
-// One workgroup: component t, words w0 .. w0+255, outputs r0 .. r0+255.
-// Waves 2x2, each 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
+// One workgroup: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words
+// w0 .. (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
+// words).  Each wave covers 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
 // One step = 64 inputs (two 32-input groups, one MFMA k-step); kw_count is a
-// multiple of 24, so the loop runs whole groups of twelve steps.  LDS holds the
-// bit masks ([group][row], 8 bytes per row and step, three buffers); each wave
+// multiple of 8, so the loop runs whole groups of four steps.  LDS holds the
+// bit masks ([group][row], 8 bytes per row and step, four buffers); each wave
 // expands its fragments in registers.  Pipeline per step s, one barrier: MFMAs
 // of step s, the mask reads and expansion of step s + 1, the masks of step
 // s + 2 into LDS and the global fetch of step s + 6.
+template <unsigned RT>
 __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ wbits, const uint32_t* __restrict__ cb,
                                                   uint32_t* __restrict__ par, size_t kw_count, size_t wc_pad,
                                                   size_t r_pad, unsigned w_tiles, unsigned r_tiles) {
-	constexpr unsigned kBufs = 3, kDepth = 4;
-	__shared__ __align__(16) uint32_t sA[kBufs][2][kBlmTile];
-	__shared__ __align__(16) uint32_t sB[kBufs][2][kBlmTile];
+	static_assert(RT == 256 || RT == 128, "row tile");
+	constexpr unsigned WT = 65536 / RT;          // words per workgroup
+	constexpr unsigned kA = 2 * WT / 256;        // word masks per thread and step
+	constexpr unsigned kB = 2 * RT / 256;        // coefficient masks per thread and step
+	constexpr unsigned kBufs = 4, kDepth = 4, kUnroll = 4;
+	__shared__ __align__(16) uint32_t sA[kBufs][2][WT];
+	__shared__ __align__(16) uint32_t sB[kBufs][2][RT];
 	const unsigned tiles = w_tiles * r_tiles;
 	const unsigned t = blockIdx.x / tiles;
 	const unsigned tile = blockIdx.x - t * tiles;
 	const unsigned wt = tile % w_tiles, rt = tile / w_tiles;
-	const size_t w0 = (size_t)wt * kBlmTile, r0 = (size_t)rt * kBlmTile;
+	const size_t w0 = (size_t)wt * WT, r0 = (size_t)rt * RT;
 	const uint32_t* A = wbits + (size_t)t * kw_count * wc_pad + w0;
 	const uint32_t* B = cb + (size_t)t * kw_count * r_pad + r0;
 	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const unsigned wr = wave >> 1, wcol = wave & 1;
+	const unsigned wr = RT == 256 ? wave >> 1 : wave, wcol = RT == 256 ? wave & 1 : 0;
 	const unsigned half = lane >> 5, row32 = lane & 31;
 
 	v16f acc[4][4];
@@ -136,12 +143,19 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 #pragma unroll
 			for(int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
 
-	// LDS layout [group][row]: a half-wave reads 32 consecutive masks
+	// LDS layout [group][row]: a half-wave reads 32 consecutive masks.
+	// m[0 .. kA) = word masks (group g = index / (kA / 2), row tid + 256 (index % (kA / 2))),
+	// m[kA .. kA + kB) = coefficient masks (RT = 256: group = index; RT = 128:
+	// group tid / 128, row tid % 128)
 	auto stage = [&](unsigned buf, const uint32_t* m) {
-		sA[buf][0][tid] = m[0];
-		sA[buf][1][tid] = m[1];
-		sB[buf][0][tid] = m[2];
-		sB[buf][1][tid] = m[3];
+#pragma unroll
+		for(unsigned q = 0; q < kA; q++) sA[buf][q / (kA / 2)][tid + 256 * (q % (kA / 2))] = m[q];
+		if(RT == 256) {
+			sB[buf][0][tid] = m[kA];
+			sB[buf][1][tid] = m[kA + 1];
+		} else {
+			sB[buf][tid / 128][tid % 128] = m[kA];
+		}
 	};
 	// fragments: this lane's mask of group `half` for its four A and four B rows
 	auto frags = [&](unsigned buf, v8i* fa, v8i* fb) {
@@ -157,18 +171,23 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 	// 32-bit element offsets from uniform bases (a plane holds < 2^32 masks)
 	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
 	auto fetch = [&](uint32_t* m, unsigned st) {
-		const unsigned ia = 2 * st * lda + tid, ib = 2 * st * ldb + tid;
-		m[0] = A[ia];
-		m[1] = A[ia + lda];
-		m[2] = B[ib];
-		m[3] = B[ib + ldb];
+		const unsigned ia = 2 * st * lda + tid;
+#pragma unroll
+		for(unsigned q = 0; q < kA; q++) m[q] = A[ia + (q / (kA / 2)) * lda + 256 * (q % (kA / 2))];
+		if(RT == 256) {
+			const unsigned ib = 2 * st * ldb + tid;
+			m[kA] = B[ib];
+			m[kA + 1] = B[ib + ldb];
+		} else {
+			m[kA] = B[(2 * st + tid / 128) * ldb + tid % 128];
+		}
 	};
 	auto clampst = [&](unsigned st) { return st < steps ? st : steps - 1; };
 	// masks of steps s + 2 .. s + 5 live in pf[(s + 2) % 4] .. (steps past the
 	// end repeat the last one: in bounds, staged but never read)
-	uint32_t pf[kDepth][4];
+	uint32_t pf[kDepth][kA + kB];
 	{
-		uint32_t m0[4], m1[4];
+		uint32_t m0[kA + kB], m1[kA + kB];
 		fetch(m0, 0);
 		fetch(m1, clampst(1));
 		stage(0, m0);
@@ -179,12 +198,12 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 	__syncthreads();
 	v8i fa[2][4], fb[2][4];
 	frags(0, fa[0], fb[0]);
-	for(unsigned s0 = 0; s0 < steps; s0 += 12) {
-		// twelve steps per pass: a common multiple of the three LDS buffers, the two
+	for(unsigned s0 = 0; s0 < steps; s0 += kUnroll) {
+		// four steps per pass: a common multiple of the four LDS buffers, the two
 		// fragment sets and the four prefetch sets, so every index is static
 #pragma unroll
-		for(unsigned k = 0; k < 12; k++) {
-			const unsigned s = s0 + k;  // steps is a multiple of 12
+		for(unsigned k = 0; k < kUnroll; k++) {
+			const unsigned s = s0 + k;  // steps is a multiple of kUnroll
 			const unsigned cur = k & 1, nxt = cur ^ 1;
 			frags((k + 1) % kBufs, fa[nxt], fb[nxt]);
 #pragma unroll
@@ -202,8 +221,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 			for(int g = 0; g < 16; g++) {
 				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
 				__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // 4 VALU
-				if(g % 4 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
-				if(g % 4 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+				if(g % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+				if(g % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
 			}
 			__syncthreads();
 		}
@@ -278,24 +297,27 @@ size_t blm_env(const char* name) {
 }
 
 struct BlmShape {
-	size_t kw_count, r_blk, wc, slot_words;
+	size_t kw_count, rt, r_blk, wc, slot_words;
 	size_t cb_bytes, wb_bytes, p_bytes, total;
 };
 
+// Row tile: 128 where it pads the outputs less than 256 does.
 BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 	BlmShape s{};
-	s.kw_count = round_up(n_in, 24 * 32) / 32;  // whole groups of twelve 64-input steps
-	s.slot_words = slot_bytes / 2;               // a multiple of 512
+	s.kw_count = round_up(n_in, kBlmKStep) / 32;  // whole groups of four 64-input steps
+	s.slot_words = slot_bytes / 2;                // a multiple of 512
+	s.rt = round_up(n_out, 128) < round_up(n_out, 256) ? 128 : 256;
+	if(size_t v = blm_env("PPH_BILINEAR_ROW_TILE")) s.rt = v == 128 ? 128 : 256;
 	const size_t per_row = kBl * s.kw_count * 4;
-	size_t rb = std::max<size_t>(kBlmTile, kBlmCoefBudget / per_row / kBlmTile * kBlmTile);
-	if(size_t cap = blm_env("PPH_BILINEAR_ROWS")) rb = std::min(rb, round_up(cap, kBlmTile));
-	s.r_blk = std::min(rb, round_up(n_out, kBlmTile));
+	size_t rb = std::max<size_t>(s.rt, kBlmCoefBudget / per_row / s.rt * s.rt);
+	if(size_t cap = blm_env("PPH_BILINEAR_ROWS")) rb = std::min(rb, round_up(cap, s.rt));
+	s.r_blk = std::min(rb, round_up(n_out, s.rt));
 	const size_t per_word = kBl * s.kw_count * 4 + kBl * s.r_blk / 8;
-	size_t wc = std::max<size_t>(kBlmTile, kBlmChunkBudget / per_word / kBlmTile * kBlmTile);
-	if(size_t cap = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(cap, kBlmTile));
-	wc = std::min(wc, round_up(s.slot_words, kBlmTile));
+	size_t wc = std::max<size_t>(kBlmWordTile, kBlmChunkBudget / per_word / kBlmWordTile * kBlmWordTile);
+	if(size_t cap = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(cap, kBlmWordTile));
+	wc = std::min(wc, round_up(s.slot_words, kBlmWordTile));
 	const size_t chunks = (s.slot_words + wc - 1) / wc;
-	s.wc = round_up((s.slot_words + chunks - 1) / chunks, kBlmTile);
+	s.wc = round_up((s.slot_words + chunks - 1) / chunks, kBlmWordTile);
 	s.cb_bytes = round_up(kBl * s.kw_count * s.r_blk * 4, 256);
 	s.wb_bytes = round_up(kBl * s.kw_count * s.wc * 4, 256);
 	s.p_bytes = round_up(kBl * s.r_blk * (s.wc / 32) * 4, 256);
@@ -322,20 +344,23 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 	auto blocks = [&](size_t threads) { return (unsigned)std::max<size_t>(1, std::min(cap, (threads + 255) / 256)); };
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
 		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
-		const size_t r_pad = round_up(rows, kBlmTile);
+		const size_t r_pad = round_up(rows, sh.rt);
 		hipLaunchKernelGGL(k_blm_coef, dim3(blocks(sh.kw_count * r_pad)), dim3(256), 0, s, cb, sh.kw_count, r_pad, n_in,
 		                   (unsigned)r0, rows, input_logs, exps, tab.exp);
 		hipError_t e = hipGetLastError();
 		if(e != hipSuccess) return set_error(e);
 		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
-			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 256
+			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 512
 			hipLaunchKernelGGL(k_blm_words, dim3(blocks(sh.kw_count * wc)), dim3(256), 0, s, wb, sh.kw_count, wc, wc,
 			                   (const uint8_t*)in_slots, slot_bytes, n_in, w0);
-			const unsigned w_tiles = (unsigned)(wc / kBlmTile), r_tiles = (unsigned)(r_pad / kBlmTile);
+			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
 			if(e == hipSuccess) e = hipGetLastError();
-			if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_gemm, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc,
-				                   r_pad, w_tiles, r_tiles);
+			if(e == hipSuccess && sh.rt == 128)
+				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par,
+				                   sh.kw_count, wc, r_pad, w_tiles, r_tiles);
+			else if(e == hipSuccess)
+				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par,
+				                   sh.kw_count, wc, r_pad, w_tiles, r_tiles);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e == hipSuccess)
 				hipLaunchKernelGGL(k_blm_combine, dim3(blocks((size_t)rows * (wc / 32))), dim3(256), 0, s, par, r_pad,

@@ -144,6 +144,24 @@ def test_bilinear_chunked(monkeypatch):
     lib().pph_bilinear_release()
 
 
+@pytest.mark.parametrize("tile", ["128", "256"])
+@pytest.mark.parametrize("n,n_out", [(257, 300), (40, 129), (520, 256)])
+def test_bilinear_row_tiles(monkeypatch, tile, n, n_out):
+    """Both GEMM row tiles (128 outputs x 512 words, 256 x 256) give the plain
+    form's bytes, with outputs and inputs off the tile and step sizes."""
+    say("bilinear row tile", tile, n, n_out)
+    S = 5000
+    rng = np.random.default_rng(n + n_out)
+    datas = [O.prng_bytes(73, n, k, S) for k in range(n)]
+    exps = rng.integers(0, 65535, n_out).astype(np.uint16)
+    lib().pph_bilinear_release()
+    monkeypatch.setenv("PPH_BILINEAR_ROW_TILE", tile)
+    got, valid, plain = run_dense(datas, [S] * n, S, exps, also_plain=True, form="bilinear")
+    lib().pph_bilinear_release()
+    assert (got == plain).all()
+    assert (valid == 1).all()
+
+
 def test_bilinear_rejects_bad_shapes():
     buf = P.DeviceBuffer(4096)
     assert lib().pph_muladd_bilinear(buf.ptr, 1000, 1, buf.ptr, buf.ptr, buf.ptr, 1, 0, None) == \
