@@ -71,77 +71,110 @@ __global__ __launch_bounds__(256) void k_blm_coef(uint32_t* __restrict__ cb, siz
 	}
 }
 
+// 32 x 32 bit-matrix transpose (rows a[j], bit b of row j -> bit j of a[b]),
+// five butterfly stages of masked swaps.
+__device__ __forceinline__ void blm_transpose32(uint32_t* a) {
+#pragma unroll
+	for(unsigned s = 16, m = 0x0000FFFFu; s; s >>= 1, m ^= m << s) {
+#pragma unroll
+		for(unsigned j = 0; j < 32; j++) {
+			if(j & s) continue;
+			const uint32_t t = ((a[j] >> s) ^ a[j + s]) & m;
+			a[j + s] ^= t;
+			a[j] ^= t << s;
+		}
+	}
+}
+
 // Word planes for words [w0, w0 + wc) of every input slot (rows padded to
-// wc_pad with zeros).  Thread: one word, one 32-input group; words vary fastest
-// so each of the 32 slot reads is coalesced.
+// wc_pad with zeros).  Thread: two adjacent words, one 32-input group; words
+// vary fastest so each of the 32 slot reads (4 bytes) is coalesced.  Row j of
+// the 32 x 32 bit matrix is input 8 (j % 4) + j / 4 (the packed position
+// order), so after the transpose row b < 16 is bit slice b of the first word
+// and row 16 + b that of the second.
 __global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits, size_t kw_count, size_t wc_pad,
                                                    size_t wc, const uint8_t* __restrict__ in_slots, size_t slot_bytes,
                                                    unsigned n_in, size_t w0) {
-	const size_t total = kw_count * wc_pad;
+	const size_t pairs = wc_pad / 2;
+	const size_t total = kw_count * pairs;
 	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const size_t kw = g / wc_pad;
-		const size_t wl = g - kw * wc_pad;
-		uint32_t s[16];
+		const size_t kw = g / pairs;
+		const size_t wl = 2 * (g - kw * pairs);
+		uint32_t a[32];
 #pragma unroll
-		for(int b = 0; b < 16; b++) s[b] = 0;
-		if(wl < wc) {
-			for(unsigned k = 0; k < 32; k++) {
-				const size_t i = kw * 32 + k;
-				if(i >= n_in) break;
-				const uint32_t x = reinterpret_cast<const uint16_t*>(in_slots + i * slot_bytes)[w0 + wl];
-				const unsigned p = blm_pos(k);
-#pragma unroll
-				for(int b = 0; b < 16; b++) s[b] |= ((x >> b) & 1u) << p;
-			}
+		for(unsigned j = 0; j < 32; j++) {
+			const size_t i = kw * 32 + 8 * (j % 4) + j / 4;
+			a[j] = (wl < wc && i < n_in)
+			           ? reinterpret_cast<const uint32_t*>(in_slots + i * slot_bytes)[(w0 + wl) / 2]
+			           : 0u;
 		}
-		uint32_t e[kBl];
-		bl_expand_slices(s, e);
+		blm_transpose32(a);
+		uint32_t e0[kBl], e1[kBl];
+		bl_expand_slices(a, e0);
+		bl_expand_slices(a + 16, e1);
 #pragma unroll
-		for(unsigned t = 0; t < kBl; t++) wbits[((size_t)t * kw_count + kw) * wc_pad + wl] = e[t];
+		for(unsigned t = 0; t < kBl; t++)
+			*reinterpret_cast<uint2*>(wbits + ((size_t)t * kw_count + kw) * wc_pad + wl) = make_uint2(e0[t], e1[t]);
 	}
 }
 
 constexpr unsigned kBlmWordTile = 512;  // word chunks are multiples of this
 constexpr unsigned kBlmKStep = 256;     // inputs are padded to multiples of this
 
-// One workgroup: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words
-// w0 .. (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
+// Work item: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words w0 ..
+// (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
 // words).  Each wave covers 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
+// The kernel is persistent: workgroup b runs items b, b + G, b + 2G, ... (G =
+// grid size), one workgroup per CU, so the launch and the first loads are paid
+// once per workgroup rather than once per item.
 // One step = 64 inputs (two 32-input groups, one MFMA k-step); kw_count is a
-// multiple of 8, so the loop runs whole groups of four steps.  LDS holds the
+// multiple of 8, so every item is whole groups of four steps.  LDS holds the
 // bit masks ([group][row], 8 bytes per row and step, four buffers); each wave
-// expands its fragments in registers.  Pipeline per step s, one barrier: MFMAs
-// of step s, the mask reads and expansion of step s + 1, the masks of step
-// s + 2 into LDS and the global fetch of step s + 6.
+// expands its operands in registers, one step after reading their masks, so
+// that no expansion waits on LDS.  One barrier per step.
+// Accumulators start at 2^23: a sum n < 2^23 then sits in the low mantissa
+// bits, and its parity is bit 0 of the float's encoding.
 template <unsigned RT>
 __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ wbits, const uint32_t* __restrict__ cb,
                                                   uint32_t* __restrict__ par, size_t kw_count, size_t wc_pad,
-                                                  size_t r_pad, unsigned w_tiles, unsigned r_tiles) {
+                                                  size_t r_pad, unsigned w_tiles, unsigned r_tiles, unsigned items) {
 	static_assert(RT == 256 || RT == 128, "row tile");
-	constexpr unsigned WT = 65536 / RT;          // words per workgroup
+	constexpr unsigned WT = 65536 / RT;          // words per item
 	constexpr unsigned kA = 2 * WT / 256;        // word masks per thread and step
 	constexpr unsigned kB = 2 * RT / 256;        // coefficient masks per thread and step
 	constexpr unsigned kBufs = 4, kDepth = 4, kUnroll = 4;
+	constexpr float kBias = 8388608.0f;          // 2^23
 	__shared__ __align__(16) uint32_t sA[kBufs][2][WT];
 	__shared__ __align__(16) uint32_t sB[kBufs][2][RT];
+	const unsigned G = gridDim.x, b = blockIdx.x;
+	const unsigned n_my = b < items ? (items - 1 - b) / G + 1 : 0;
+	if(!n_my) return;
 	const unsigned tiles = w_tiles * r_tiles;
-	const unsigned t = blockIdx.x / tiles;
-	const unsigned tile = blockIdx.x - t * tiles;
-	const unsigned wt = tile % w_tiles, rt = tile / w_tiles;
-	const size_t w0 = (size_t)wt * WT, r0 = (size_t)rt * RT;
-	const uint32_t* A = wbits + (size_t)t * kw_count * wc_pad + w0;
-	const uint32_t* B = cb + (size_t)t * kw_count * r_pad + r0;
+	const unsigned steps = (unsigned)(kw_count / 2);
 	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const unsigned wr = RT == 256 ? wave >> 1 : wave, wcol = RT == 256 ? wave & 1 : 0;
 	const unsigned half = lane >> 5, row32 = lane & 31;
+	// 32-bit element offsets from uniform per-item bases (a plane holds < 2^32 masks)
+	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
+	struct Item {
+		unsigned t, w0, r0;
+	};
+	auto item = [&](unsigned j) {
+		const unsigned it = b + j * G;
+		const unsigned t = it / tiles, tile = it - t * tiles;
+		return Item{t, (tile % w_tiles) * WT, (tile / w_tiles) * RT};
+	};
 
 	v16f acc[4][4];
+	auto reset = [&] {
 #pragma unroll
-	for(int i = 0; i < 4; i++)
+		for(int i = 0; i < 4; i++)
 #pragma unroll
-		for(int j = 0; j < 4; j++)
+			for(int j = 0; j < 4; j++)
 #pragma unroll
-			for(int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
+				for(int v = 0; v < 16; v++) acc[i][j][v] = kBias;
+	};
+	reset();
 
 	// LDS layout [group][row]: a half-wave reads 32 consecutive masks.
 	// m[0 .. kA) = word masks (group g = index / (kA / 2), row tid + 256 (index % (kA / 2))),
@@ -157,93 +190,150 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 			sB[buf][tid / 128][tid % 128] = m[kA];
 		}
 	};
-	// fragments: this lane's mask of group `half` for its four A and four B rows
-	auto frags = [&](unsigned buf, v8i* fa, v8i* fb) {
+	// this lane's masks (group `half`) of its four A rows and four B rows, then
+	// their e2m1 expansion into MFMA operands
+	auto read_masks = [&](unsigned buf, uint32_t* rm) {
 #pragma unroll
 		for(int i = 0; i < 4; i++) {
-			const uint4 qa = blm_nibbles(sA[buf][half][128 * wr + 32 * i + row32]);
-			const uint4 qb = blm_nibbles(sB[buf][half][128 * wcol + 32 * i + row32]);
+			rm[i] = sA[buf][half][128 * wr + 32 * i + row32];
+			rm[4 + i] = sB[buf][half][128 * wcol + 32 * i + row32];
+		}
+	};
+	auto expand = [&](const uint32_t* rm, v8i* fa, v8i* fb) {
+#pragma unroll
+		for(int i = 0; i < 4; i++) {
+			const uint4 qa = blm_nibbles(rm[i]);
+			const uint4 qb = blm_nibbles(rm[4 + i]);
 			fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
 			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
 		}
 	};
-	const unsigned steps = (unsigned)(kw_count / 2);
-	// 32-bit element offsets from uniform bases (a plane holds < 2^32 masks)
-	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
-	auto fetch = [&](uint32_t* m, unsigned st) {
-		const unsigned ia = 2 * st * lda + tid;
+	// Per item: the masks of steps 0 .. 6 come in (issued before the previous
+	// item's epilogue, so that their latency hides behind it), steps 0 .. 2 go
+	// to LDS, then the step loop.  Fetches past the item's last step repeat the
+	// last one (in bounds, never used).
+	struct Src {
+		const uint32_t *a, *b;
+	};
+	auto base = [&](unsigned j) {
+		const Item x = item(j);
+		return Src{wbits + (size_t)x.t * kw_count * wc_pad + x.w0, cb + (size_t)x.t * kw_count * r_pad + x.r0};
+	};
+	auto fetch = [&](uint32_t* m, const Src& f, unsigned st) {
+		const unsigned ls = st < steps ? st : steps - 1;
+		const unsigned ia = 2 * ls * lda + tid;
 #pragma unroll
-		for(unsigned q = 0; q < kA; q++) m[q] = A[ia + (q / (kA / 2)) * lda + 256 * (q % (kA / 2))];
+		for(unsigned q = 0; q < kA; q++) m[q] = f.a[ia + (q / (kA / 2)) * lda + 256 * (q % (kA / 2))];
 		if(RT == 256) {
-			const unsigned ib = 2 * st * ldb + tid;
-			m[kA] = B[ib];
-			m[kA + 1] = B[ib + ldb];
+			const unsigned ib = 2 * ls * ldb + tid;
+			m[kA] = f.b[ib];
+			m[kA + 1] = f.b[ib + ldb];
 		} else {
-			m[kA] = B[(2 * st + tid / 128) * ldb + tid % 128];
+			m[kA] = f.b[(2 * ls + tid / 128) * ldb + tid % 128];
 		}
 	};
-	auto clampst = [&](unsigned st) { return st < steps ? st : steps - 1; };
-	// masks of steps s + 2 .. s + 5 live in pf[(s + 2) % 4] .. (steps past the
-	// end repeat the last one: in bounds, staged but never read)
-	uint32_t pf[kDepth][kA + kB];
-	{
-		uint32_t m0[kA + kB], m1[kA + kB];
-		fetch(m0, 0);
-		fetch(m1, clampst(1));
-		stage(0, m0);
-		stage(1, m1);
-	}
+	// Pipeline, step s (buffer and register-set indices static in the unrolled
+	// pass): MFMAs on the operands of step s; expansion of the masks of step
+	// s + 1 (read from LDS during step s - 1); LDS reads of the masks of step
+	// s + 2; the global masks of step s + 3 into LDS; the global fetch of step
+	// s + 7.
+	uint32_t m0[3][kA + kB], pf[kDepth][kA + kB];
+	Src cur = base(0);
 #pragma unroll
-	for(unsigned k = 0; k < kDepth; k++) fetch(pf[(2 + k) % kDepth], clampst(2 + k));
-	__syncthreads();
-	v8i fa[2][4], fb[2][4];
-	frags(0, fa[0], fb[0]);
-	for(unsigned s0 = 0; s0 < steps; s0 += kUnroll) {
-		// four steps per pass: a common multiple of the four LDS buffers, the two
-		// fragment sets and the four prefetch sets, so every index is static
+	for(unsigned k = 0; k < 3; k++) fetch(m0[k], cur, k);
 #pragma unroll
-		for(unsigned k = 0; k < kUnroll; k++) {
-			const unsigned s = s0 + k;  // steps is a multiple of kUnroll
-			const unsigned cur = k & 1, nxt = cur ^ 1;
-			frags((k + 1) % kBufs, fa[nxt], fb[nxt]);
-#pragma unroll
-			for(int i = 0; i < 4; i++)
-#pragma unroll
-				for(int j = 0; j < 4; j++)
-					acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[cur][i], fb[cur][j], acc[i][j], 4, 4,
-					                                                            0, 127, 0, 127);
-			stage((k + 2) % kBufs, pf[(k + 2) % kDepth]);
-			fetch(pf[(k + 2) % kDepth], clampst(s + 2 + kDepth));
-			// order: mask reads first, then MFMAs interleaved with the expansion,
-			// LDS writes and global loads
-			__builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS reads
-#pragma unroll
-			for(int g = 0; g < 16; g++) {
-				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-				__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // 4 VALU
-				if(g % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
-				if(g % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
-			}
-			__syncthreads();
-		}
-	}
-	// parity bits: lane (half h, row32 = output column) holds words 8(v/4) + 4h + v%4
+	for(unsigned k = 0; k < kDepth; k++) fetch(pf[(3 + k) % kDepth], cur, 3 + k);
 	const size_t wbk = wc_pad / 32;
+	for(unsigned cj = 0; cj < n_my; cj++) {
 #pragma unroll
-	for(int i = 0; i < 4; i++)
+		for(unsigned k = 0; k < 3; k++) stage(k, m0[k]);
+		__syncthreads();
+		v8i fa[2][4], fb[2][4];
+		uint32_t rm[2][8];
+		read_masks(0, rm[0]);
+		expand(rm[0], fa[0], fb[0]);
+		read_masks(1, rm[1]);
+		for(unsigned s0 = 0; s0 < steps; s0 += kUnroll) {
+			// four steps per pass: a common multiple of the four LDS buffers, the two
+			// operand and mask sets and the four prefetch sets
 #pragma unroll
-		for(int j = 0; j < 4; j++) {
-			uint32_t m = 0;
+			for(unsigned k = 0; k < kUnroll; k++) {
+				const unsigned s = s0 + k;
+				const unsigned c = k & 1, n = c ^ 1;
+				read_masks((k + 2) % kBufs, rm[c]);
+				expand(rm[n], fa[n], fb[n]);
 #pragma unroll
-			for(int v = 0; v < 16; v++)
-				m |= ((uint32_t)(int)acc[i][j][v] & 1u) << (8 * (v >> 2) + 4 * half + (v & 3));
-			m |= __shfl_xor(m, 32);
-			if(!half) {
-				const size_t r = r0 + 128 * wcol + 32 * j + row32;
-				const size_t wb = (w0 + 128 * wr + 32 * i) / 32;
-				par[((size_t)t * r_pad + r) * wbk + wb] = m;
+				for(int i = 0; i < 4; i++)
+#pragma unroll
+					for(int j = 0; j < 4; j++)
+						acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[c][i], fb[c][j], acc[i][j], 4, 4,
+						                                                            0, 127, 0, 127);
+				stage((k + 3) % kBufs, pf[(k + 3) % kDepth]);
+				fetch(pf[(k + 3) % kDepth], cur, s + 3 + kDepth);
+				// LDS reads first, then MFMAs interleaved with the expansion (whose
+				// masks arrived a step ago), the LDS writes and the global loads.  The
+				// 128-row tile (five global loads and LDS writes per step) does better
+				// with its reads split around the first MFMAs.
+				if(RT == 256) {
+					__builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS reads
+#pragma unroll
+					for(int q = 0; q < 16; q++) {
+						__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+						__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // 4 VALU
+						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+					}
+				} else {
+					__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+					__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+					__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+					__builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+#pragma unroll
+					for(int q = 0; q < 13; q++) {
+						__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+						__builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+					}
+				}
+				__syncthreads();
 			}
 		}
+		// the next item's first masks (the last item fetches its own again)
+		const Item x = item(cj);
+		cur = base(cj + 1 < n_my ? cj + 1 : cj);
+#pragma unroll
+		for(unsigned k = 0; k < 3; k++) fetch(m0[k], cur, k);
+#pragma unroll
+		for(unsigned k = 0; k < kDepth; k++) fetch(pf[(3 + k) % kDepth], cur, 3 + k);
+		// item cj done: parity bits, lane (half h, row32 = output column) holding
+		// words 8(v/4) + 4h + v%4 of each block; both half-waves hold the whole
+		// mask and store the same word (no divergent branch in the item loop).
+		// The sums are read with an AGPR-constrained move, which keeps the
+		// accumulators in AGPRs across the item loop (a plain VALU read inside
+		// the loop makes the allocator move them to VGPRs and spill).  A VALU
+		// read of an MFMA result needs 19 wait states after a 16-pass MFMA;
+		// the padding gives 24, as the asm reads are outside the compiler's
+		// hazard tracking.
+		asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7");
+#pragma unroll
+		for(int i = 0; i < 4; i++)
+#pragma unroll
+			for(int j = 0; j < 4; j++) {
+				uint32_t m = 0;
+#pragma unroll
+				for(int v = 0; v < 16; v++) {
+					uint32_t u;
+					asm("v_accvgpr_read_b32 %0, %1" : "=v"(u) : "a"(acc[i][j][v]));
+					m |= (u & 1u) << (8 * (v >> 2) + 4 * half + (v & 3));
+				}
+				m |= __shfl_xor(m, 32);
+				const size_t r = x.r0 + 128 * wcol + 32 * j + row32;
+				const size_t wb = (x.w0 + 128 * wr + 32 * i) / 32;
+				par[((size_t)x.t * r_pad + r) * wbk + wb] = m;
+			}
+		reset();
+	}
 }
 
 // Recovery words from the parity bits: thread = (output row, 32-word block),
@@ -351,16 +441,18 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 		if(e != hipSuccess) return set_error(e);
 		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
 			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 512
-			hipLaunchKernelGGL(k_blm_words, dim3(blocks(sh.kw_count * wc)), dim3(256), 0, s, wb, sh.kw_count, wc, wc,
+			hipLaunchKernelGGL(k_blm_words, dim3(blocks(sh.kw_count * wc / 2)), dim3(256), 0, s, wb, sh.kw_count, wc, wc,
 			                   (const uint8_t*)in_slots, slot_bytes, n_in, w0);
 			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
+			const unsigned items = kBl * w_tiles * r_tiles;
+			const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e == hipSuccess && sh.rt == 128)
-				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par,
-				                   sh.kw_count, wc, r_pad, w_tiles, r_tiles);
+				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc, r_pad,
+				                   w_tiles, r_tiles, items);
 			else if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(kBl * w_tiles * r_tiles), dim3(256), 0, s, wb, cb, par,
-				                   sh.kw_count, wc, r_pad, w_tiles, r_tiles);
+				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc, r_pad,
+				                   w_tiles, r_tiles, items);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e == hipSuccess)
 				hipLaunchKernelGGL(k_blm_combine, dim3(blocks((size_t)rows * (wc / 32))), dim3(256), 0, s, par, r_pad,
