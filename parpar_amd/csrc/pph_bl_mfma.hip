@@ -208,8 +208,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
 		}
 	};
-	// Per item: the masks of steps 0 .. 6 come in (issued before the previous
-	// item's epilogue, so that their latency hides behind it), steps 0 .. 2 go
+	// Per item: the masks of steps 0 .. 7 come in (issued before the previous
+	// item's epilogue, so that their latency hides behind it), steps 0 .. 3 go
 	// to LDS, then the step loop.  Fetches past the item's last step repeat the
 	// last one (in bounds, never used).
 	struct Src {
@@ -235,24 +235,26 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 	// Pipeline, step s (buffer and register-set indices static in the unrolled
 	// pass): MFMAs on the operands of step s; expansion of the masks of step
 	// s + 1 (read from LDS during step s - 1); LDS reads of the masks of step
-	// s + 2; the global masks of step s + 3 into LDS; the global fetch of step
-	// s + 7.
-	uint32_t m0[3][kA + kB], pf[kDepth][kA + kB];
+	// s + 2; the global masks of step s + 4 into LDS; the global fetch of step
+	// s + 8.  A barrier after every second step: a buffer is written two steps
+	// after its last read and read two steps after its write.
+	uint32_t m0[4][kA + kB], pf[kDepth][kA + kB];
 	Src cur = base(0);
 #pragma unroll
-	for(unsigned k = 0; k < 3; k++) fetch(m0[k], cur, k);
+	for(unsigned k = 0; k < 4; k++) fetch(m0[k], cur, k);
 #pragma unroll
-	for(unsigned k = 0; k < kDepth; k++) fetch(pf[(3 + k) % kDepth], cur, 3 + k);
+	for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], cur, 4 + k);
 	const size_t wbk = wc_pad / 32;
 	for(unsigned cj = 0; cj < n_my; cj++) {
 #pragma unroll
-		for(unsigned k = 0; k < 3; k++) stage(k, m0[k]);
+		for(unsigned k = 0; k < 4; k++) stage(k, m0[k]);
 		__syncthreads();
 		v8i fa[2][4], fb[2][4];
 		uint32_t rm[2][8];
 		read_masks(0, rm[0]);
 		expand(rm[0], fa[0], fb[0]);
 		read_masks(1, rm[1]);
+		__syncthreads();  // buffers 0 and 1 are written again in the first two steps
 		for(unsigned s0 = 0; s0 < steps; s0 += kUnroll) {
 			// four steps per pass: a common multiple of the four LDS buffers, the two
 			// operand and mask sets and the four prefetch sets
@@ -268,8 +270,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 					for(int j = 0; j < 4; j++)
 						acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[c][i], fb[c][j], acc[i][j], 4, 4,
 						                                                            0, 127, 0, 127);
-				stage((k + 3) % kBufs, pf[(k + 3) % kDepth]);
-				fetch(pf[(k + 3) % kDepth], cur, s + 3 + kDepth);
+				stage((k + 4) % kBufs, pf[k]);
+				fetch(pf[k], cur, s + 4 + kDepth);
 				// LDS reads first, then MFMAs interleaved with the expansion (whose
 				// masks arrived a step ago), the LDS writes and the global loads.  The
 				// 128-row tile (five global loads and LDS writes per step) does better
@@ -296,16 +298,16 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
 					}
 				}
-				__syncthreads();
+				if(k & 1) __syncthreads();
 			}
 		}
 		// the next item's first masks (the last item fetches its own again)
 		const Item x = item(cj);
 		cur = base(cj + 1 < n_my ? cj + 1 : cj);
 #pragma unroll
-		for(unsigned k = 0; k < 3; k++) fetch(m0[k], cur, k);
+		for(unsigned k = 0; k < 4; k++) fetch(m0[k], cur, k);
 #pragma unroll
-		for(unsigned k = 0; k < kDepth; k++) fetch(pf[(3 + k) % kDepth], cur, 3 + k);
+		for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], cur, 4 + k);
 		// item cj done: parity bits, lane (half h, row32 = output column) holding
 		// words 8(v/4) + 4h + v%4 of each block; both half-waves hold the whole
 		// mask and store the same word (no divergent branch in the item loop).
@@ -320,12 +322,17 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 		for(int i = 0; i < 4; i++)
 #pragma unroll
 			for(int j = 0; j < 4; j++) {
+				uint32_t u[16];
+#pragma unroll
+				for(int v = 0; v < 16; v++) asm("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
+				// sum v = 4q + c goes to bit 8q + 4h + c: gather the low bytes of sums
+				// c, 4 + c, 8 + c, 12 + c into bytes 0..3, keep bit 0 of each, shift
 				uint32_t m = 0;
 #pragma unroll
-				for(int v = 0; v < 16; v++) {
-					uint32_t u;
-					asm("v_accvgpr_read_b32 %0, %1" : "=v"(u) : "a"(acc[i][j][v]));
-					m |= (u & 1u) << (8 * (v >> 2) + 4 * half + (v & 3));
+				for(int c = 0; c < 4; c++) {
+					const uint32_t w = __builtin_amdgcn_perm(u[4 + c], u[c], 0x0c0c0400u) |
+					                   __builtin_amdgcn_perm(u[12 + c], u[8 + c], 0x04000c0cu);
+					m |= (w & 0x01010101u) << (4 * half + c);
 				}
 				m |= __shfl_xor(m, 32);
 				const size_t r = x.r0 + 128 * wcol + 32 * j + row32;
