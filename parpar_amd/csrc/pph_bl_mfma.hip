@@ -314,17 +314,18 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 		// The sums are read with an AGPR-constrained move, which keeps the
 		// accumulators in AGPRs across the item loop (a plain VALU read inside
 		// the loop makes the allocator move them to VGPRs and spill).  A VALU
-		// read of an MFMA result needs 19 wait states after a 16-pass MFMA;
-		// the padding gives 24, as the asm reads are outside the compiler's
-		// hazard tracking.
-		asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7");
+		// read of an MFMA result needs up to 19 wait states after the MFMA, and
+		// asm reads are outside the compiler's hazard tracking: the padding gives
+		// 32, and every read is volatile so that none is scheduled above it
+		// (volatile asm statements keep their order).
+		asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7");
 #pragma unroll
 		for(int i = 0; i < 4; i++)
 #pragma unroll
 			for(int j = 0; j < 4; j++) {
 				uint32_t u[16];
 #pragma unroll
-				for(int v = 0; v < 16; v++) asm("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
+				for(int v = 0; v < 16; v++) asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
 				// sum v = 4q + c goes to bit 8q + 4h + c: gather the low bytes of sums
 				// c, 4 + c, 8 + c, 12 + c into bytes 0..3, keep bit 0 of each, shift
 				uint32_t m = 0;
