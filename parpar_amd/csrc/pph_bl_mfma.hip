@@ -1,14 +1,15 @@
 // Bilinear form on fp4 MFMA: the 81 integer GEMMs of the rank-81 formula
 // (see pph_bilinear.hip) computed by a hand-written kernel on
-// v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (0 / 1.0, unit scales;
-// the fp32 sums are exact integers up to 2^24), twice the int8 rate.
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (0 or a power of two,
+// paired so that every product of set elements is 1; unit scales; the fp32
+// sums are exact integers up to 2^24), twice the int8 rate.
 //
 // Operands are bit-packed planes, one bit per element:
 //   Cb[t][kw][r]  bit of E(c(r, 32kw + k))_t     (coefficients, outputs r)
 //   Wb[t][kw][w]  bit of E(x_{32kw + k}[w])_t     (input words w)
-// with element k of a 32-input group at bit 4(k % 8) + k / 8, so that nibble
-// dword q of the fp4 expansion is ((m >> q) & 0x11111111) << 1.  The GEMM
-// kernel (one workgroup per 256 words x 256 outputs, or 512 words x 128
+// with element k of a 32-input group at bit 4(k % 8) + q(k / 8) (blm_pos,
+// blm_pos_c), so that most operand dwords are one AND of the mask.  The GEMM
+// kernel (persistent; items of 256 words x 256 outputs, or 512 words x 128
 // outputs, and component t) stages them in LDS (one 64-input step per
 // iteration), runs 16 MFMAs per wave per step and writes the parity of each
 // sum as a bit:
@@ -29,13 +30,24 @@ namespace pph {
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-// bit position of element k (0..31) of a 32-input group in a packed mask
+// Packed masks: element k (0..31) of a 32-input group sits at bit 4 (k % 8)
+// + q of its mask, q a per-operand function of k / 8, so that the MFMA
+// operand (nibble k % 8 of dword k / 8) is mostly one AND.  A set nibble bit
+// is an e2m1 value (bit 0: 0.5, bit 1: 1.0, bit 2: 2.0); the word operand
+// puts groups 0, 1, 2, 3 on bits 0, 1, 2, 2 and the coefficient operand on
+// bits 2, 1, 0, 0, so every product of two set elements is exactly 1 and the
+// fp32 sums are the integer counts.
+//   word masks:        q = k / 8           (dword 3 shifted down one bit)
+//   coefficient masks: q = (2, 1, 0, 3)[k / 8]  (dword 3 shifted down three)
 __device__ __forceinline__ unsigned blm_pos(unsigned k) { return 4 * (k & 7) + (k >> 3); }
+__device__ __forceinline__ unsigned blm_pos_c(unsigned k) { return 4 * (k & 7) + ((0x3012u >> (4 * (k >> 3))) & 15u); }
 
-// 32 packed elements -> 16 bytes of e2m1 nibbles (0 or 1.0 = 0b0010)
+// 32 packed elements -> 16 bytes of e2m1 nibbles
 __device__ __forceinline__ uint4 blm_nibbles(uint32_t m) {
-	return make_uint4((m & 0x11111111u) << 1, ((m >> 1) & 0x11111111u) << 1, ((m >> 2) & 0x11111111u) << 1,
-	                  ((m >> 3) & 0x11111111u) << 1);
+	return make_uint4(m & 0x11111111u, m & 0x22222222u, m & 0x44444444u, (m >> 1) & 0x44444444u);
+}
+__device__ __forceinline__ uint4 blm_nibbles_c(uint32_t m) {
+	return make_uint4(m & 0x44444444u, m & 0x22222222u, m & 0x11111111u, (m >> 3) & 0x11111111u);
 }
 
 // Coefficient planes for outputs r0 .. r0 + n_rows (rows padded to r_pad with
@@ -59,7 +71,7 @@ __global__ __launch_bounds__(256) void k_blm_coef(uint32_t* __restrict__ cb, siz
 				if(i >= n_in) break;
 				uint32_t l = mod65535((uint32_t)in_logs[i] * er);
 				const uint32_t c = exp_tab[l == kOrder ? 0u : l];
-				const unsigned p = blm_pos(k);
+				const unsigned p = blm_pos_c(k);
 #pragma unroll
 				for(int b = 0; b < 16; b++) s[b] |= ((c >> b) & 1u) << p;
 			}
@@ -203,7 +215,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 #pragma unroll
 		for(int i = 0; i < 4; i++) {
 			const uint4 qa = blm_nibbles(rm[i]);
-			const uint4 qb = blm_nibbles(rm[4 + i]);
+			const uint4 qb = blm_nibbles_c(rm[4 + i]);
 			fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
 			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
 		}
@@ -212,25 +224,41 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 	// item's epilogue, so that their latency hides behind it), steps 0 .. 3 go
 	// to LDS, then the step loop.  Fetches past the item's last step repeat the
 	// last one (in bounds, never used).
+	// Loads are buffer loads: a resource per item and plane (scalar), a byte
+	// offset per thread and load that never changes, and the step's uniform
+	// byte offset in a scalar register, so the address costs no VALU work.
+	// A plane is < 2^31 bytes (the workspace chunking keeps it so); reads past
+	// the resource's range return 0.
 	struct Src {
-		const uint32_t *a, *b;
+		__amdgpu_buffer_rsrc_t a, b;
+		unsigned w0, r0;
 	};
+	constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit access, gfx9 family
+	const int plane_a = (int)(kw_count * wc_pad * 4), plane_b = (int)(kw_count * r_pad * 4);
 	auto base = [&](unsigned j) {
 		const Item x = item(j);
-		return Src{wbits + (size_t)x.t * kw_count * wc_pad + x.w0, cb + (size_t)x.t * kw_count * r_pad + x.r0};
+		return Src{__builtin_amdgcn_make_buffer_rsrc((void*)(wbits + (size_t)x.t * kw_count * wc_pad), 0, plane_a,
+		                                             kRsrcWord3),
+		           __builtin_amdgcn_make_buffer_rsrc((void*)(cb + (size_t)x.t * kw_count * r_pad), 0, plane_b,
+		                                             kRsrcWord3),
+		           x.w0, x.r0};
 	};
+	int voff[kA + kB];
+#pragma unroll
+	for(unsigned q = 0; q < kA; q++) voff[q] = (int)(((q / (kA / 2)) * lda + 256 * (q % (kA / 2)) + tid) * 4);
+	if(RT == 256) {
+		voff[kA] = (int)(tid * 4);
+		voff[kA + 1] = (int)((ldb + tid) * 4);
+	} else {
+		voff[kA] = (int)(((tid / 128) * ldb + tid % 128) * 4);
+	}
 	auto fetch = [&](uint32_t* m, const Src& f, unsigned st) {
 		const unsigned ls = st < steps ? st : steps - 1;
-		const unsigned ia = 2 * ls * lda + tid;
+		const int sa = (int)((2 * ls * lda + f.w0) * 4), sb = (int)((2 * ls * ldb + f.r0) * 4);
 #pragma unroll
-		for(unsigned q = 0; q < kA; q++) m[q] = f.a[ia + (q / (kA / 2)) * lda + 256 * (q % (kA / 2))];
-		if(RT == 256) {
-			const unsigned ib = 2 * ls * ldb + tid;
-			m[kA] = f.b[ib];
-			m[kA + 1] = f.b[ib + ldb];
-		} else {
-			m[kA] = f.b[(2 * ls + tid / 128) * ldb + tid % 128];
-		}
+		for(unsigned q = 0; q < kA; q++) m[q] = __builtin_amdgcn_raw_buffer_load_b32(f.a, voff[q], sa, 0);
+#pragma unroll
+		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, voff[kA + q], sb, 0);
 	};
 	// Pipeline, step s (buffer and register-set indices static in the unrolled
 	// pass): MFMAs on the operands of step s; expansion of the masks of step
@@ -435,6 +463,8 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 	if(!n_out) return PPH_OK;
 	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out);
 	if(!ws || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
+	// the GEMM addresses one plane of a component through a buffer resource
+	if(sh.kw_count * sh.wc * 4 >= (1ull << 31) || sh.kw_count * sh.r_blk * 4 >= (1ull << 31)) return PPH_E_INVALID;
 	uint32_t* cb = static_cast<uint32_t*>(ws);
 	uint32_t* wb = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes);
 	uint32_t* par = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes + sh.wb_bytes);
