@@ -237,7 +237,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep(uint8_t* slots, const uin
 			ck_rows<CK, kPrepThreads, 2>(src, slot, nrows, nb, len, h, s_log, run_end);
 		else
 			ck_rows<CK, kPrepThreads, 1>(src, slot, nrows, nb, len, h, s_log, run_end);
-		__threadfence();  // row rewrites complete before the checksum block is stored over them
+		// row rewrites complete before the checksum block is stored over them: all
+		// of them come from this workgroup (one CU, one L2), so a workgroup barrier
+		// orders them; a device-scope fence would write back the whole L2 per slot
+		__syncthreads();
 		uint32_t lane = ck_reduce<CK, kPrepThreads>(h, nb, run_end, lg, ex);
 		if(threadIdx.x < CK / 2) {
 			uint16_t v = (uint16_t)lane;
@@ -266,7 +269,10 @@ __global__ __launch_bounds__(kCkThreads) void k_prep_data(uint8_t* slots, const 
 		uint32_t h[CkGeom<CK, kCkThreads>::DW];
 		size_t run_end;
 		ck_rows<CK, kCkThreads, 1>(src, slot, nrows, nb, len, h, nullptr, run_end);
-		__threadfence();  // row rewrites complete before the checksum block is stored over them
+		// row rewrites complete before the checksum block is stored over them: all
+		// of them come from this workgroup (one CU, one L2), so a workgroup barrier
+		// orders them; a device-scope fence would write back the whole L2 per slot
+		__syncthreads();
 		uint32_t lane = ck_reduce<CK, kCkThreads>(h, nb, run_end, lg, ex);
 		if(threadIdx.x < CK / 2) *reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = (uint16_t)lane;
 	}
