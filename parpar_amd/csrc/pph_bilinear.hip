@@ -56,33 +56,11 @@ static void bl_expand_host(uint32_t v, unsigned n, std::vector<uint8_t>& out) {
 	bl_expand_host(lo ^ hi, h, out);
 }
 
-// Products of expanded operands -> coefficients of the polynomial product.
-static uint64_t bl_combine_host(const uint8_t* p, size_t count, unsigned n) {
-	if(n == 1) return p[0] & 1;
-	const unsigned h = n / 2;
-	const size_t k = count / 3;
-	const uint64_t L = bl_combine_host(p, k, h), H = bl_combine_host(p + k, k, h), D = bl_combine_host(p + 2 * k, k, h);
-	return L ^ ((D ^ L ^ H) << h) ^ (H << n);
-}
-
-static uint32_t bl_reduce_host(uint64_t v) {
-	for(int b = 30; b >= 16; b--)
-		if((v >> b) & 1) v ^= (uint64_t)kPoly << (b - 16);
-	return (uint32_t)v;
-}
-
-// Recombination words: W_t = Red(combine(unit t)).
+// Recombination words W_t (the compile-time table of pph_karatsuba.h): the
+// product is the XOR of W_t over the components whose parity is 1.
 static const uint16_t* bl_recombine_table() {
-	static uint16_t w[kBl];
-	static std::once_flag once;
-	std::call_once(once, [] {
-		for(unsigned t = 0; t < kBl; t++) {
-			uint8_t p[kBl] = {};
-			p[t] = 1;
-			w[t] = (uint16_t)bl_reduce_host(bl_combine_host(p, kBl, 16));
-		}
-	});
-	return w;
+	static constexpr BlTable kW = bl_table();
+	return kW.w;
 }
 
 // ------------------------------------------------------------------ device
@@ -356,9 +334,11 @@ extern "C" uint16_t pph_bilinear_mul(uint16_t c, uint16_t x) {
 	std::vector<uint8_t> ec, ex;
 	bl_expand_host(c, 16, ec);
 	bl_expand_host(x, 16, ex);
-	uint8_t p[kBl];
-	for(unsigned t = 0; t < kBl; t++) p[t] = ec[t] & ex[t];
-	return (uint16_t)bl_reduce_host(bl_combine_host(p, kBl, 16));
+	const uint16_t* w = bl_recombine_table();
+	uint16_t r = 0;
+	for(unsigned t = 0; t < kBl; t++)
+		if(ec[t] & ex[t]) r ^= w[t];
+	return r;
 }
 
 extern "C" int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes) {

@@ -8,6 +8,33 @@ namespace pph {
 
 constexpr unsigned kBl = 81;  // rank of the four-level Karatsuba formula
 
+// Recombination of component t: the product polynomial of the unit vector e_t
+// (Karatsuba recombination L + (D + L + H) x^h + H x^n per level, components
+// in the order of bl_expand_slices), reduced modulo the PAR2 polynomial
+// 0x1100B.  W_t is a 16-bit mask: bit j set where component t's parity enters
+// bit j of the product.  Evaluated at compile time.
+constexpr uint64_t bl_combine_unit(unsigned t, unsigned count, unsigned n) {
+	if(n == 1) return 1;
+	const unsigned h = n / 2, k = count / 3;
+	const uint64_t L = t < k ? bl_combine_unit(t, k, h) : 0;
+	const uint64_t H = t >= k && t < 2 * k ? bl_combine_unit(t - k, k, h) : 0;
+	const uint64_t D = t >= 2 * k ? bl_combine_unit(t - 2 * k, k, h) : 0;
+	return L ^ ((D ^ L ^ H) << h) ^ (H << n);
+}
+constexpr uint16_t bl_reduce(uint64_t v) {
+	for(int b = 30; b >= 16; b--)
+		if((v >> b) & 1) v ^= (uint64_t)0x1100B << (b - 16);
+	return (uint16_t)v;
+}
+struct BlTable {
+	uint16_t w[kBl];
+};
+constexpr BlTable bl_table() {
+	BlTable r{};
+	for(unsigned t = 0; t < kBl; t++) r.w[t] = bl_reduce(bl_combine_unit(t, kBl, 16));
+	return r;
+}
+
 // Bit-sliced expansion: s[b] holds bit b of 32 values (value k at a bit position
 // of the caller's choosing); e[t] receives component t of the same 32 values at
 // the same positions.  Components in recursive order: E(v) = E(lo) ++ E(hi) ++
