@@ -77,6 +77,8 @@ double plain_rate(const struct ::pph_tuning& tu, unsigned n_out);
 int autotune(struct ::pph_tuning* out);
 
 // Bilinear form (pph_bilinear.hip); int status codes.
+// true when PPH_BILINEAR_GEMM=library selects the hipBLASLt GEMMs (not capturable)
+bool bl_library();
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
