@@ -285,6 +285,89 @@ def main_serving(args):
     print(json.dumps(res), flush=True)
 
 
+def main_backend(args):
+    """The plain workload through the drop-in boundary: PAR2ProcHIP (pph_proc_*)
+    fed host buffers the way ParPar's controller does (addInput per slice,
+    endInput, getOutput per recovery slice), so PCIe both ways is inside the
+    timed region.  Not the headline line: the device-resident rate is."""
+    import numpy as np
+    import parpar_amd
+    from parpar_amd import proc as P
+    import oracle as O
+
+    parpar_amd.require_device()
+    L = parpar_amd.lib()
+    chk = parpar_amd.check
+    n, R, S = args.inputs, args.recovery, args.slice
+    ma = n * R * (S // 2)
+    log("backend workload: %d inputs x %d B, %d recovery slices, MA=%.4g (host buffers)" % (n, S, R, ma))
+    # host inputs: the device generator's bytes, copied down once
+    inputs = np.empty((n, S), np.uint8)
+    chunk = max(1, (256 << 20) // S)
+    buf = P.DeviceBuffer(chunk * S)
+    for k0 in range(0, n, chunk):
+        c = min(chunk, n - k0)
+        chk(L.pph_fill_random(buf.ptr, S, c, S, args.seed, 0, k0, None))
+        chk(L.pph_synchronize())
+        chk(L.pph_memcpy_d2h(P._ptr(inputs[k0:k0 + c]), buf.ptr, c * S))
+    buf.close()
+    out = np.zeros((R, S), np.uint8)
+    be = P.PAR2ProcHIP(0, 2)
+    be.setSliceSize(S)
+    be.init(args.cksum, args.batch, parpar_amd.FORM_PLAIN)
+    be.setRecoverySlices(np.arange(R, dtype=np.uint16))
+
+    def step():
+        be.discardOutput()
+        for i in range(n):
+            if be.canAdd() == parpar_amd.PPH_ADD_FULL:
+                be.waitForAdd()
+            be.addInput(inputs[i], S, inputNum=i)
+        be.endInput()
+        ok = True
+        for r in range(R):
+            ok = be.getOutput(r, out[r]) and ok
+        return ok
+
+    for _ in range(args.warmup):
+        step()
+    times, valid = [], True
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        valid = step() and valid
+        times.append(time.perf_counter() - t0)
+    be.close()
+    ms = 1e3 * float(np.mean(times))
+    rate = 2.0 * ma / (ms / 1e3) / 1e9
+    log("backend: %.1f ms per pass, %.1f GB/s mul-add-equivalent, flags %s" % (ms, rate, valid))
+    # CPU check on a column window of every input and recovery slice
+    win = min(4096, S)
+    tc = time.perf_counter()
+    ref = O.recover(np.ascontiguousarray(inputs[:, :win]), [win] * n, win, np.arange(R, dtype=np.uint16),
+                    threads=cpu_threads())
+    t_cpu = time.perf_counter() - tc
+    ok = bool((out[:, :win] == ref).all()) and valid
+    res = {
+        "metric": "GF16 mul-add GB/s (2 B per GF(2^16) mul-add) through the PAR2ProcHIP boundary, host buffers",
+        "value": round(rate, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u16 (GF(2^16) words)",
+        "data": "synthetic (counter-based random bytes, seed %d), in host memory" % args.seed,
+        "config": {"workload": "backend pass: %d host inputs x %d B -> %d recovery slices (exp 0..%d), input batch "
+                               "%d, addInput/endInput/getOutput with PCIe both ways" % (
+                                   n, S, R, R - 1, be_batch(args)), "inputs": n, "slice_bytes": S, "recovery": R},
+        "cpu_check": {"window_bytes": win, "matches_gpu": ok, "mul_adds": n * R * (win // 2),
+                      "seconds": round(t_cpu, 2)},
+    }
+    print(json.dumps(res), flush=True)
+    if not ok:
+        raise SystemExit("backend output differs from the oracle")
+
+
+def be_batch(args):
+    return args.batch if args.batch else 64
+
+
 def main_batched(args):
     import numpy as np
     import parpar_amd
@@ -414,7 +497,8 @@ def parse():
     ap.add_argument("--slice", type=int, default=None)
     ap.add_argument("--recovery", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving"], default=None)
+    ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving", "backend"], default=None)
+    ap.add_argument("--batch", type=int, default=0, help="backend: input batch size (0 = the backend's default)")
     ap.add_argument("--window", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
     ap.add_argument("--bilinear-workspace", type=int, default=40,
@@ -509,6 +593,8 @@ def main():
         return main_batched(args)
     if args.config == 4:
         return main_serving(args)
+    if args.form == "backend":
+        return main_backend(args)
 
     import numpy as np
     import parpar_amd
