@@ -1,12 +1,15 @@
 // C ABI of libparpar_hip.so: device-layer entry points and the pph_proc
 // recovery backend (the IPAR2ProcBackend contract, gf16/controller.h:77-217).
 #include "pph_internal.h"
+#include "pph_copy_pool.h"
 #include "../../include/parpar_hip.h"
 
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -393,13 +396,14 @@ extern "C" int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned c
 // isActive, gf16/controller.h:45-75).
 struct Area {
 	uint8_t* d_in = nullptr;        // batch * slot_bytes
-	uint8_t* h_stage = nullptr;     // pinned, batch * slice bytes
+	uint8_t* h_stage = nullptr;     // pinned, batch * slot_bytes (the device slots' layout)
 	uint32_t* h_meta = nullptr;     // pinned: lens[batch] | slot idx[batch] | in_log[batch] (u16 packed)
 	uint32_t* d_meta = nullptr;
 	uint32_t* h_coef = nullptr;     // pinned explicit table batch * r_pad (or null)
 	uint32_t* d_coef = nullptr;
 	hipEvent_t done = nullptr;
 	bool active = false;
+	unsigned h2d_first = 0, h2d_count = 0;  // staged slots whose copy to the device is not issued yet
 	bool explicit_coef = false;
 	unsigned prep_count = 0;        // slots added since last submit needing preparation
 };
@@ -428,6 +432,9 @@ struct pph_proc {
 	unsigned active_count = 0;
 	unsigned batches_started = 0;
 	bool processing_add = false;
+	std::unique_ptr<pph::CopyPool> copier;  // host copies into and out of pinned memory
+	uint8_t* h_bounce = nullptr;            // pinned, one slice: getOutput's device-to-host copy
+	size_t bounce_bytes = 0;
 };
 
 namespace {
@@ -487,10 +494,10 @@ int wait_area(pph_proc* p, Area& a) {
 int alloc_areas(pph_proc* p) {
 	free_areas(p);
 	p->areas.resize(p->n_areas);
-	const size_t stage = p->slice_size ? p->slice_size : 1;
 	for(auto& a : p->areas) {
 		if(hipMalloc(&a.d_in, (size_t)p->batch * p->slot_bytes) != hipSuccess) return PPH_E_NOMEM;
-		if(hipHostMalloc(&a.h_stage, (size_t)p->batch * stage, hipHostMallocDefault) != hipSuccess) return PPH_E_NOMEM;
+		if(hipHostMalloc(&a.h_stage, (size_t)p->batch * p->slot_bytes, hipHostMallocDefault) != hipSuccess)
+			return PPH_E_NOMEM;
 		const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
 		if(hipHostMalloc(&a.h_meta, meta, hipHostMallocDefault) != hipSuccess) return PPH_E_NOMEM;
 		if(hipMalloc(&a.d_meta, meta) != hipSuccess) return PPH_E_NOMEM;
@@ -516,8 +523,24 @@ int ensure_coef_tables(pph_proc* p) {
 }
 
 // submit the current area's `n` inputs (PAR2ProcOCL::run_kernel, controller_ocl.cpp:496-527)
+// Staged slots go to the device in runs of up to kH2DRun slots, one contiguous
+// copy each (the staging has the device slots' layout; bytes past a slice's
+// length are zeroed by the preparation): one DMA per 1 MiB slice runs well
+// below the link rate.
+constexpr unsigned kH2DRun = 16;
+
+int flush_h2d(pph_proc* p, Area& a) {
+	if(!a.h2d_count) return PPH_OK;
+	const size_t off = (size_t)a.h2d_first * p->slot_bytes;
+	PPH_HIP(hipMemcpyAsync(a.d_in + off, a.h_stage + off, (size_t)a.h2d_count * p->slot_bytes, hipMemcpyHostToDevice,
+	                       p->s_copy));
+	a.h2d_count = 0;
+	return PPH_OK;
+}
+
 int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
 	Area& a = p->areas[area_idx];
+	if(int rc = flush_h2d(p, a)) return rc;
 	uint32_t *lens, *slots;
 	uint16_t* logs;
 	meta_ptrs(p, a, &lens, &slots, &logs);
@@ -623,11 +646,17 @@ int add_common(pph_proc* p, const void* buffer, size_t size, const uint16_t* coe
 		set_coeffs_num(p, a, idx, input_num);
 	}
 	if(!dummy) {
-		uint8_t* stage = a.h_stage + (size_t)idx * p->alloc_slice;
-		if(size) memcpy(stage, buffer, size);
-		if(size)
-			PPH_HIP(hipMemcpyAsync(a.d_in + (size_t)idx * p->slot_bytes, stage, size, hipMemcpyHostToDevice,
-			                       p->s_copy));
+		uint8_t* stage = a.h_stage + (size_t)idx * p->slot_bytes;
+		if(size) p->copier->copy(stage, buffer, size);
+		if(a.h2d_count && a.h2d_first + a.h2d_count != idx) {
+			rc = flush_h2d(p, a);
+			if(rc) return rc;
+		}
+		if(!a.h2d_count) a.h2d_first = idx;
+		if(++a.h2d_count == kH2DRun) {
+			rc = flush_h2d(p, a);
+			if(rc) return rc;
+		}
 		lens[a.prep_count] = (uint32_t)size;
 		slots[a.prep_count] = idx;
 		a.prep_count++;
@@ -669,6 +698,14 @@ extern "C" int pph_proc_create(int device, unsigned staging_areas, pph_proc** ou
 		pph_proc_destroy(p);
 		return rc;
 	}
+	// helper threads for the staging copies (PPH_COPY_THREADS, default 3)
+	unsigned helpers = 3;
+	if(const char* v = getenv("PPH_COPY_THREADS")) helpers = (unsigned)std::min(15l, std::max(0l, strtol(v, nullptr, 10)));
+	p->copier.reset(new(std::nothrow) CopyPool(helpers));
+	if(!p->copier) {
+		pph_proc_destroy(p);
+		return PPH_E_NOMEM;
+	}
 	*out = p;
 	return PPH_OK;
 }
@@ -683,6 +720,7 @@ extern "C" void pph_proc_destroy(pph_proc* p) {
 	if(p->ev_copied) (void)hipEventDestroy(p->ev_copied);
 	if(p->s_copy) (void)hipStreamDestroy(p->s_copy);
 	if(p->s_comp) (void)hipStreamDestroy(p->s_comp);
+	if(p->h_bounce) (void)hipHostFree(p->h_bounce);
 	delete p;
 }
 
@@ -817,8 +855,9 @@ extern "C" int pph_proc_fill_input(pph_proc* p, const void* buffer) {
 	Area& a = p->areas[p->cur_area];
 	int rc = wait_area(p, a);
 	if(rc) return rc;
-	uint8_t* stage = a.h_stage + (size_t)p->cur_inputs * p->alloc_slice;
-	memcpy(stage, buffer, p->cur_size);
+	if((rc = flush_h2d(p, a))) return rc;
+	uint8_t* stage = a.h_stage + (size_t)p->cur_inputs * p->slot_bytes;
+	p->copier->copy(stage, buffer, p->cur_size);
 	uint8_t* dst = a.d_in + (size_t)p->cur_inputs * p->slot_bytes;
 	PPH_HIP(hipMemcpyAsync(dst, stage, p->cur_size, hipMemcpyHostToDevice, p->s_copy));
 	PPH_HIP(launch_prep(dst, nullptr, p->slot_bytes, 1, nullptr, nullptr, p->cur_size, ck_offset(p->cur_size), p->ck,
@@ -871,8 +910,27 @@ extern "C" int pph_proc_get_output(pph_proc* p, unsigned index, void* output, in
 		PPH_HIP(hipStreamSynchronize(p->s_comp));
 		p->verified = true;
 	}
-	if(p->cur_size)
-		PPH_HIP(hipMemcpy(output, p->d_out + (size_t)index * p->slot_bytes, p->cur_size, hipMemcpyDeviceToHost));
+	if(p->cur_size) {
+		// through a pinned bounce slice (full-rate DMA) and a parallel host copy
+		if(p->bounce_bytes < p->cur_size) {
+			if(p->h_bounce) (void)hipHostFree(p->h_bounce);
+			p->h_bounce = nullptr;
+			p->bounce_bytes = 0;
+			if(hipHostMalloc((void**)&p->h_bounce, p->alloc_slice, hipHostMallocDefault) == hipSuccess)
+				p->bounce_bytes = p->alloc_slice;
+			else
+				p->h_bounce = nullptr;
+			(void)hipGetLastError();
+		}
+		const uint8_t* src = p->d_out + (size_t)index * p->slot_bytes;
+		if(p->h_bounce) {
+			PPH_HIP(hipMemcpyAsync(p->h_bounce, src, p->cur_size, hipMemcpyDeviceToHost, p->s_comp));
+			PPH_HIP(hipStreamSynchronize(p->s_comp));
+			p->copier->copy(output, p->h_bounce, p->cur_size);
+		} else {
+			PPH_HIP(hipMemcpy(output, src, p->cur_size, hipMemcpyDeviceToHost));
+		}
+	}
 	if(valid) *valid = p->h_valid[index];
 	return PPH_OK;
 }

@@ -1,5 +1,7 @@
-"""The serving engine's host MD5 (RFC 1321) against hashlib: every padding
-boundary near 64-byte blocks, and input fed in odd pieces."""
+"""Host-side helpers of the engine, built with g++ from their headers: the
+serving engine's MD5 (RFC 1321) against hashlib, and the backend's parallel
+host copy.  MD5: the RFC 1321 vectors, every padding boundary near 64-byte
+blocks, and input fed in odd pieces."""
 import hashlib
 import os
 import shutil
@@ -42,3 +44,14 @@ def test_lengths_and_pieces(md5_tool, n):
     want = hashlib.md5(data).hexdigest()
     assert digest(md5_tool, data) == want
     assert digest(md5_tool, data, [1, 63, 2, 64, 130, 7]) == want
+
+
+def test_backend_copy_pool(tmp_path):
+    """The backend's parallel host copy (staging into pinned memory)."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not present")
+    exe = str(tmp_path / "copy_pool_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", os.path.join(HERE, "cpp", "copy_pool_check.cpp"), "-o",
+                    exe], check=True)
+    r = subprocess.run([exe], capture_output=True, check=True, timeout=120)
+    assert r.stdout.decode().strip() == "ok"
