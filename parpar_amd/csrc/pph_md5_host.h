@@ -18,15 +18,24 @@ class HostMd5 {
 		return (x << C) | (x >> (32 - C));
 	}
 	// One step: a = b + rotl(a + f(b, c, d) + K[i] + m[g], s), with the four
-	// round functions of RFC 1321 section 3.4 (F and G as bit selects).
+	// round functions of RFC 1321 section 3.4.  Only b is produced by the
+	// previous step, so everything that does not read b is summed into t
+	// first: a + K + m, and for G = (b & d) | (c & ~d) the disjoint half
+	// c & ~d (the halves share no bits, so | is +).  The empty asm pins that
+	// sum; without it the compiler reassociates K after f and puts an extra
+	// add on the dependent chain.  The chain per step is then the ops of f
+	// that read b (2, 1, 1, 2 for F, G, H, I), one add, the rotate and the
+	// final add.
 	template <unsigned R, unsigned C>
 	static void step(uint32_t& a, uint32_t b, uint32_t c, uint32_t d, uint32_t km) {
-		uint32_t f;
-		if(R == 0) f = d ^ (b & (c ^ d));
-		else if(R == 1) f = c ^ (d & (b ^ c));
-		else if(R == 2) f = b ^ c ^ d;
-		else f = c ^ (b | ~d);
-		a = b + rotl<C>(a + f + km);
+		uint32_t t = a + km;
+		if(R == 1) t += c & ~d;
+		asm("" : "+r"(t));
+		if(R == 0) t += d ^ (b & (c ^ d));
+		else if(R == 1) t += b & d;
+		else if(R == 2) t += b ^ (c ^ d);
+		else t += c ^ (b | ~d);
+		a = b + rotl<C>(t);
 	}
 
 	void block(const uint8_t* p) {

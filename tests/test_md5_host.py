@@ -1,7 +1,8 @@
-"""Host-side helpers of the engine, built with g++ from their headers: the
-serving engine's MD5 (RFC 1321) against hashlib, and the backend's parallel
-host copy.  MD5: the RFC 1321 vectors, every padding boundary near 64-byte
-blocks, and input fed in odd pieces."""
+"""Host-side helpers of the engine, built from their headers: the serving
+engine's MD5 (RFC 1321) against hashlib, and the backend's parallel host copy.
+MD5: the RFC 1321 vectors, every padding boundary near 64-byte blocks, and
+input fed in odd pieces; built with g++ and with ROCm's clang (the compiler of
+the engine's host code), since the step order is pinned per compiler."""
 import hashlib
 import os
 import shutil
@@ -13,12 +14,16 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.fixture(scope="module")
-def md5_tool(tmp_path_factory):
-    if shutil.which("g++") is None:
-        pytest.skip("g++ not present")
+ROCM_CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
+@pytest.fixture(scope="module", params=[("g++", "-O2"), (ROCM_CLANG, "-O3")], ids=["gcc", "rocm-clang"])
+def md5_tool(request, tmp_path_factory):
+    cxx, opt = request.param
+    if shutil.which(cxx) is None:
+        pytest.skip(cxx + " not present")
     exe = str(tmp_path_factory.mktemp("md5") / "md5_host_check")
-    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "cpp", "md5_host_check.cpp"), "-o", exe],
+    subprocess.run([cxx, opt, "-std=c++17", os.path.join(HERE, "cpp", "md5_host_check.cpp"), "-o", exe],
                    check=True)
     return exe
 
