@@ -143,12 +143,11 @@ def main_serving(args):
         return (k * 0x9E3779B1 * 4096 + 12345) % (len(pool) - T + 1)
 
     srv = SV.Server(budget, max_compute=args.max_compute, bilinear_workspace=bl_ws,
-                    md5_host_threads=args.md5_host_threads if args.host_data else 0,
+                    md5_host_threads=args.md5_host_threads,
                     upload_threads=args.upload_threads if args.host_data else 0)
-    log("arena %.1f GiB, %d compute streams, bilinear workspace %d GiB%s" % (
-        budget / 2**30, args.max_compute, args.bilinear_workspace,
-        ", host data: %d MD5 threads, %d upload threads" % (args.md5_host_threads, args.upload_threads)
-        if args.host_data else ""))
+    log("arena %.1f GiB, %d compute streams, bilinear workspace %d GiB, %d file-MD5 host threads%s" % (
+        budget / 2**30, args.max_compute, args.bilinear_workspace, args.md5_host_threads,
+        ", host data: %d upload threads" % args.upload_threads if args.host_data else ""))
     # warm-up job (tables, first launches), not counted
     srv.run_all([SV.FileJob(8 << 20, 1 << 20, 4, seed=args.seed, job_id=1 << 30)])
 
@@ -256,7 +255,9 @@ def main_serving(args):
         "value": round(achieved, 2), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(makespan * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u16 (GF(2^16) words); MD5 / CRC-32 bytes",
-        "data": ("synthetic (counter-based random bytes generated on the device, seed %d)" % args.seed
+        "data": ("synthetic (counter-based random bytes generated on the device, seed %d); %s" % (
+                     args.seed, "file MD5 on %d host threads, read back from HBM" % args.md5_host_threads
+                     if args.md5_host_threads else "file MD5 in shared device rounds")
                  if pool is None else
                  "synthetic, host-resident: each file a window of a %.0f GiB host pool of random bytes, "
                  "copied through pinned staging by %d threads; file MD5 on %d host threads" % (
@@ -264,7 +265,8 @@ def main_serving(args):
         "config": {"workload": "serving: Poisson 20 files/s for %.0f s, file 1 MiB..8 GiB, slice 256 KiB..16 MiB, "
                                "recovery 1..30%% of slices" % args.window, "config_index": 4,
                    "files": len(arrivals), "file_bytes": total_bytes, "mul_adds": total_ma,
-                   "arena_bytes": budget, "compute_streams": args.max_compute, "bilinear_workspace": bl_ws,
+                   "arena_bytes": budget, "compute_streams": args.max_compute,
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default"), "bilinear_workspace": bl_ws,
                    "device": name.value.decode(), "cus": cus.value},
         "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
@@ -506,8 +508,12 @@ def parse():
     ap.add_argument("--host-data", action="store_true",
                     help="serving: files in host memory (pinned staging uploads, file MD5 on host threads)")
     ap.add_argument("--pool-gib", type=int, default=16, help="serving --host-data: host pool size")
-    ap.add_argument("--md5-host-threads", type=int, default=12)
+    ap.add_argument("--md5-host-threads", type=int, default=12,
+                    help="serving: host threads for the file MD5 (0: shared hash rounds on the device)")
     ap.add_argument("--upload-threads", type=int, default=4)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="serving: GPU_MAX_HW_QUEUES for this process (>= max-compute + 1 gives every compute "
+                         "stream and the hash stream a hardware queue of its own; 0 leaves the environment as is)")
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
@@ -592,6 +598,11 @@ def main():
     if args.config == 2:
         return main_batched(args)
     if args.config == 4:
+        if not 0 <= args.hw_queues <= 32:
+            sys.exit("--hw-queues must be in 0..32")
+        if args.hw_queues:
+            # read by the HIP runtime when it initialises, which nothing above has done yet
+            os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
         return main_serving(args)
     if args.form == "backend":
         return main_backend(args)

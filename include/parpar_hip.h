@@ -289,10 +289,11 @@ typedef struct pph_serve_config {
 	                               budget (0: plain form only).  Jobs whose predicted bilinear time is
 	                               lower and whose workspace fits use it, one job's GEMMs at a time. */
 	const struct pph_tuning* tuning;  /* rates for that choice; NULL runs pph_autotune at create */
-	unsigned md5_host_threads;  /* > 0: this many host threads compute the file MD5 of jobs whose bytes are in
-	                               host memory, each file from submit on, beside the device work (one core
-	                               runs an MD5 chain several times faster than one wavefront); 0: every
-	                               file MD5 on the device */
+	unsigned md5_host_threads;  /* > 0: this many host threads compute every file MD5, beside the device work
+	                               (one core runs an MD5 chain several times faster than one wavefront):
+	                               host_data files from submit on, device-generated files once their
+	                               inputs are in HBM, read back through two pinned chunks per thread of
+	                               min(staging_bytes, 4 MiB); 0: every file MD5 on the device */
 	unsigned upload_threads;    /* threads copying host-sourced inputs to the device, each with its own two
 	                               pinned staging chunks (0 = 1, at most 16) */
 } pph_serve_config;

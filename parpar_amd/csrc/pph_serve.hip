@@ -19,9 +19,10 @@
 // whose inputs are on the device joins a shared hash round instead: one
 // kernel on the hash stream, one wavefront per file, each advancing its chain
 // by at most `md5_quantum` bytes.  Rounds repeat while chains are unfinished;
-// files join and leave at round boundaries.  Optionally, files whose bytes
-// are in host memory are hashed by a pool of host threads instead, from
-// submit on (a core runs the chain several times faster than a wavefront).
+// files join and leave at round boundaries.  Optionally, a pool of host
+// threads hashes every file instead (a core runs the chain several times
+// faster than a wavefront): bytes in host memory from submit on, bytes
+// generated in HBM once they are there, read back through pinned chunks.
 //
 // A job's memory is two arena ranges: the inputs and hash results (held until
 // its MD5 chain ends) and the log-domain inputs and recovery slices (released
@@ -95,6 +96,7 @@ constexpr unsigned kMaxBypass = 16;   // admissions that may pass a waiting head
 constexpr unsigned kLookBehind = 8;   // queued jobs considered behind the head
 constexpr unsigned kMaxSpans = 1024;  // files per hash round
 constexpr unsigned kMaxUploaders = 16;
+constexpr size_t kMd5ReadChunk = 4u << 20;  // pinned read-back chunk of a host MD5 thread
 
 struct Job {
 	pph_serve_job spec;
@@ -121,6 +123,7 @@ struct Job {
 	uint64_t md5_pos = 0;    // bytes of the chain dispatched
 	bool md5_done = false;   // for host_md5 jobs: set by the hashing thread, guarded by the engine mutex
 	bool host_md5 = false;   // file MD5 on a host thread (fixed at submit)
+	bool host_queued = false;  // handed to the host MD5 threads (guarded by the engine mutex)
 	uint8_t host_digest[16] = {};
 	int status = PPH_OK;
 };
@@ -158,6 +161,7 @@ struct pph_serve {
 	std::deque<Round> rounds;  // in flight, oldest first
 	unsigned next_round_buf = 0;
 	std::vector<Stager> stagers;  // one per upload thread
+	std::vector<Stager> md5_stagers;  // one per host MD5 thread, allocated at its first device-resident file
 	void* bl_ws = nullptr;               // bilinear workspace
 	hipEvent_t bl_tail = nullptr;        // end of the last enqueued bilinear phase
 	bool bl_any = false;
@@ -170,7 +174,8 @@ struct pph_serve {
 	std::vector<Job*> active;  // admitted, not finished
 	std::deque<Job*> uploads;
 	std::deque<Job*> deliveries;
-	std::deque<Job*> host_md5;  // submitted, file MD5 not yet started on a host thread
+	std::deque<Job*> host_md5;  // file MD5 not yet started on a host thread: host bytes from submit,
+	                            // device-resident bytes once they are in HBM
 	std::vector<pph_serve_result> done;
 	pph_serve_stats stats{};
 	uint64_t next_ticket = 1;
@@ -524,6 +529,12 @@ void scheduler(pph_serve* s) {
 			if(!j->hashing && hipEventQuery(j->ev_data) == hipSuccess) {
 				j->hashing = true;
 				progress = true;
+				if(j->host_md5 && !j->spec.host_data) {
+					std::lock_guard<std::mutex> lk(s->mu);
+					j->host_queued = true;
+					s->host_md5.push_back(j);
+					s->cv.notify_all();
+				}
 			}
 			if(!j->delivering && hipEventQuery(j->ev_done) == hipSuccess) {
 				deliver(s, j);
@@ -540,14 +551,15 @@ void scheduler(pph_serve* s) {
 		// keep two rounds queued on the hash stream
 		while(s->rounds.size() < 2 && launch_round(s, hashing)) progress = true;
 		for(Job* j : act) {
-			bool computed, md5_done;
+			bool computed, md5_done, host_queued;
 			{
 				std::lock_guard<std::mutex> lk(s->mu);
 				computed = j->computed;
 				md5_done = j->md5_done;
+				host_queued = j->host_queued;
 			}
 			// a host thread may still read a failed job's bytes: it ends only after its hash
-			if(computed && (md5_done || (j->status != PPH_OK && !j->host_md5))) {
+			if(computed && (md5_done || (j->status != PPH_OK && !host_queued))) {
 				finish_job(s, j);
 				progress = true;
 			}
@@ -589,8 +601,50 @@ void uploader(pph_serve* s, unsigned k) {
 	}
 }
 
-// File MD5 of host-sourced jobs, oldest submit first.  No device calls.
-void md5_worker(pph_serve* s) {
+// File MD5 of a job whose inputs are in HBM: its slices are read back in
+// file order through the thread's two pinned chunks on the hash stream, the
+// copy of the next piece in flight while the current one is hashed.  The
+// job's input memory stays allocated until its MD5 is done.
+int hash_device_file(pph_serve* s, Job* j, Stager& st, HostMd5& h) {
+	const size_t chunk = std::min<size_t>(s->cfg.staging_bytes, kMd5ReadChunk);
+	hipError_t e = hipSuccess;
+	for(int b = 0; b < 2 && e == hipSuccess; b++) {
+		if(!st.buf[b]) e = hipHostMalloc((void**)&st.buf[b], chunk, hipHostMallocDefault);
+		if(e == hipSuccess && !st.ev[b]) e = hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming);
+	}
+	const size_t S = j->spec.slice_size;
+	size_t k = 0, off = 0, n[2] = {0, 0};
+	auto issue = [&](unsigned b) {  // the next piece (within one slice) into chunk b
+		const size_t len = k + 1 < j->n_in ? S : j->last_len;
+		n[b] = std::min(chunk, len - off);
+		hipError_t r =
+		    hipMemcpyAsync(st.buf[b], j->data + k * j->slot + off, n[b], hipMemcpyDeviceToHost, s->hash_stream);
+		if(r == hipSuccess) r = hipEventRecord(st.ev[b], s->hash_stream);
+		off += n[b];
+		if(off == len) {
+			k++;
+			off = 0;
+		}
+		return r;
+	};
+	unsigned b = 0;
+	if(e == hipSuccess) e = issue(0);
+	while(e == hipSuccess) {
+		const bool more = k < j->n_in;
+		if(more) e = issue(b ^ 1u);  // chunk b^1 was hashed in the previous pass
+		if(e == hipSuccess) e = hipEventSynchronize(st.ev[b]);
+		if(e != hipSuccess) break;
+		h.update(st.buf[b], n[b]);
+		if(!more) break;
+		b ^= 1u;
+	}
+	return e == hipSuccess ? PPH_OK : set_error(e);
+}
+
+// File MD5 on host thread k: host-sourced files from submit on, files
+// generated in HBM once their inputs are there; oldest first.
+void md5_worker(pph_serve* s, unsigned k) {
+	bool device_set = false;
 	for(;;) {
 		Job* j = nullptr;
 		{
@@ -601,9 +655,16 @@ void md5_worker(pph_serve* s) {
 			s->host_md5.pop_front();
 		}
 		HostMd5 h;
-		h.update((const uint8_t*)j->spec.host_data, (size_t)j->spec.file_size);
+		int rc = PPH_OK;
+		if(j->spec.host_data) {
+			h.update((const uint8_t*)j->spec.host_data, (size_t)j->spec.file_size);
+		} else {
+			if(!device_set) device_set = hipSetDevice(s->device) == hipSuccess;
+			rc = device_set ? hash_device_file(s, j, s->md5_stagers[k], h) : PPH_E_DEVICE;
+		}
 		h.final(j->host_digest);
 		std::lock_guard<std::mutex> lk(s->mu);
+		if(rc != PPH_OK && j->status == PPH_OK) j->status = rc;
 		j->md5_done = true;
 		s->cv.notify_all();
 	}
@@ -639,11 +700,12 @@ extern "C" void pph_serve_destroy(pph_serve* s) {
 		(void)hipStreamSynchronize(s->hash_stream);
 		(void)hipStreamDestroy(s->hash_stream);
 	}
-	for(Stager& st : s->stagers)
-		for(int k = 0; k < 2; k++) {
-			if(st.ev[k]) (void)hipEventDestroy(st.ev[k]);
-			if(st.buf[k]) (void)hipHostFree(st.buf[k]);
-		}
+	for(auto* group : {&s->stagers, &s->md5_stagers})
+		for(Stager& st : *group)
+			for(int k = 0; k < 2; k++) {
+				if(st.ev[k]) (void)hipEventDestroy(st.ev[k]);
+				if(st.buf[k]) (void)hipHostFree(st.buf[k]);
+			}
 	for(int k = 0; k < 2; k++) {
 		if(s->round_ev[k]) (void)hipEventDestroy(s->round_ev[k]);
 		if(s->h_spans[k]) (void)hipHostFree(s->h_spans[k]);
@@ -692,6 +754,7 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 		if(e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
 	if(e == hipSuccess) e = hipStreamCreateWithFlags(&s->hash_stream, hipStreamNonBlocking);
 	s->stagers.resize(c.upload_threads);
+	s->md5_stagers.resize(c.md5_host_threads);
 	for(Stager& st : s->stagers)
 		for(int k = 0; k < 2 && e == hipSuccess; k++) {
 			e = hipHostMalloc((void**)&st.buf[k], c.staging_bytes, hipHostMallocDefault);
@@ -726,7 +789,7 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	s->stats.arena_bytes = c.hbm_budget;
 	s->sched = std::thread(scheduler, s);
 	for(unsigned k = 0; k < c.upload_threads; k++) s->uploaders.emplace_back(uploader, s, k);
-	for(unsigned k = 0; k < c.md5_host_threads; k++) s->md5_threads.emplace_back(md5_worker, s);
+	for(unsigned k = 0; k < c.md5_host_threads; k++) s->md5_threads.emplace_back(md5_worker, s, k);
 	s->deliverer = std::thread(deliverer, s);
 	*out = s;
 	return PPH_OK;
@@ -767,7 +830,8 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 		delete j;
 		return PPH_E_NOMEM;  // could never be admitted
 	}
-	j->host_md5 = s->cfg.md5_host_threads && job->host_data;
+	j->host_md5 = s->cfg.md5_host_threads != 0;
+	j->host_queued = j->host_md5 && job->host_data;
 	j->t_arrival = now_s();
 	std::lock_guard<std::mutex> lk(s->mu);
 	if(s->stop) {
@@ -777,7 +841,7 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 	j->ticket = s->next_ticket++;
 	if(ticket) *ticket = j->ticket;
 	s->pending.push_back(j);
-	if(j->host_md5) s->host_md5.push_back(j);
+	if(j->host_queued) s->host_md5.push_back(j);
 	s->outstanding++;
 	s->stats.submitted++;
 	s->cv.notify_all();

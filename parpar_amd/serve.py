@@ -94,9 +94,9 @@ class Server:
                  tuning=None, md5_host_threads=0, upload_threads=0):
         """bilinear_workspace > 0 lets large jobs use the bilinear form; tuning
         (parpar_amd.batch.PPHTuning) gives the rates that choose it, None
-        measures them at create.  md5_host_threads > 0 hashes files given as
-        host bytes on that many host threads; upload_threads copy host bytes
-        to the device (0 = 1)."""
+        measures them at create.  md5_host_threads > 0 computes every file MD5
+        on that many host threads (device-generated files read back from
+        HBM); upload_threads copy host bytes to the device (0 = 1)."""
         self._tuning = tuning  # kept alive while the engine reads it
         self.cfg = ServeConfig(int(hbm_budget), int(max_compute), int(md5_quantum), int(staging_bytes),
                                int(bilinear_workspace),

@@ -146,20 +146,34 @@ def test_serve_bilinear_form():
 
 
 def test_serve_host_md5_and_uploaders():
-    """Host-sourced files hashed on host threads and copied by several upload
-    threads (small staging chunks, many copies per slice) give the same bytes
-    as device hashing, alone and mixed."""
+    """File MD5 on host threads (host-sourced files from host memory,
+    device-generated files read back from HBM in small pinned chunks, several
+    pieces per slice) and host bytes copied by several upload threads give the
+    same bytes as device hashing, alone and mixed."""
     jobs = mixed_jobs()
-    n_host = sum(1 for j in jobs if j.data is not None)
-    say("serve host md5", n_host)
+    say("serve host md5", len(jobs))
     with SV.Server(hbm_budget=256 * MiB, max_compute=4, staging_bytes=12288, md5_host_threads=3,
                    upload_threads=3) as srv:
         srv.run_all(jobs)
         st = srv.stats()
     say(" stats", st)
-    assert st["host_md5_jobs"] == n_host
+    assert st["host_md5_jobs"] == len(jobs) and st["md5_rounds"] == 0
     for job in jobs:
         check_job(job)
+
+    # device-generated file, hashed on the device and on a host thread
+    def generated():
+        return SV.FileJob(13 * 20000 + 44, 20000, 3, seed=21, job_id=9, first_input=5)
+
+    on_dev, on_host = generated(), generated()
+    with SV.Server(hbm_budget=64 * MiB, max_compute=2, md5_quantum=4096) as srv:
+        srv.run_all([on_dev])
+    with SV.Server(hbm_budget=64 * MiB, max_compute=2, md5_host_threads=2, staging_bytes=7000) as srv:
+        srv.run_all([on_host] + mixed_jobs()[:3])
+        assert srv.stats()["md5_rounds"] == 0
+    for name in ("recovery", "valid", "slice_md5", "slice_crc", "file_md5"):
+        assert (getattr(on_dev, name) == getattr(on_host, name)).all(), name
+    check_job(on_host)
 
     def target():
         fs = 11 * 16384 + 5
