@@ -204,7 +204,8 @@ uint16_t pph_bilinear_mul(uint16_t c, uint16_t x);
 /* Per-slice hashes of the input path (hasher/hasher_base.h:83-198): for slot k,
  * MD5 (md5_out + 16k) and CRC-32 (crc_out[k]) of bytes [0, lens[k]) zero-padded
  * to slice_size (lens: device array, NULL = slice_size; slice_size % 4 == 0).
- * One lane per slice.  Run on data-form slots, before log-domain preparation. */
+ * MD5: one lane per slice; CRC-32: up to 64 lanes per slice, combined.  Run on
+ * data-form slots (16-byte aligned), before log-domain preparation. */
 int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens, size_t slice_size,
                    uint8_t* md5_out, uint32_t* crc_out, void* stream);
 /* MD5 of a file laid out over `count` slots, slice_size bytes each (the last

@@ -35,14 +35,14 @@ struct HostTables {
 	uint16_t log[65536];
 	uint16_t in_log[32768];
 	uint16_t rowmask[16 * 64];
-	uint32_t crc[4 * 256];
+	uint32_t crc[16 * 256];  // slice-by-16: table t maps a byte followed by t zero bytes
 	HostTables() {
 		for(uint32_t i = 0; i < 256; i++) {
 			uint32_t c = i;
 			for(int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : (c >> 1);
 			crc[i] = c;
 		}
-		for(int tab = 1; tab < 4; tab++)
+		for(int tab = 1; tab < 16; tab++)
 			for(uint32_t i = 0; i < 256; i++)
 				crc[tab * 256 + i] = (crc[(tab - 1) * 256 + i] >> 8) ^ crc[crc[(tab - 1) * 256 + i] & 255];
 		uint32_t n = 1;

@@ -120,61 +120,82 @@ __device__ __forceinline__ uint32_t keep4(size_t pos, size_t len) {
 
 // ------------------------------------------------------------------ per slice
 
-// Lane k hashes slot k: bytes [0, lens[k]) followed by zeros up to slice_size.
-// CRC-32 by 4-byte steps through the slice-by-4 tables in LDS.
-__global__ __launch_bounds__(64) void k_slice_hash(const uint8_t* __restrict__ slots, size_t slot_bytes, unsigned count,
-                                                   const uint32_t* __restrict__ lens, size_t slice_size,
-                                                   uint8_t* __restrict__ md5_out, uint32_t* __restrict__ crc_out,
-                                                   const uint32_t* __restrict__ crc_tab) {
-	__shared__ uint32_t s_crc[4][256];
-	for(unsigned k = threadIdx.x; k < 1024; k += 64) s_crc[k >> 8][k & 255] = crc_tab[k];
-	__syncthreads();
-	const unsigned k = blockIdx.x * 64 + threadIdx.x;
-	if(k >= count) return;
+// The 64-byte block at byte `base` of a message of len bytes, zero past len.
+__device__ __forceinline__ void load_block(const uint32_t* __restrict__ src, size_t base, size_t len, uint32_t m[16]) {
+	if(base + 64 <= len) {
+#pragma unroll
+		for(int w = 0; w < 16; w += 4) {
+			const uint4 v = *reinterpret_cast<const uint4*>(src + base / 4 + w);
+			m[w] = v.x;
+			m[w + 1] = v.y;
+			m[w + 2] = v.z;
+			m[w + 3] = v.w;
+		}
+	} else if(base >= len) {
+#pragma unroll
+		for(int w = 0; w < 16; w++) m[w] = 0;
+	} else {
+#pragma unroll
+		for(int w = 0; w < 16; w++) m[w] = src[base / 4 + w] & keep4(base + 4 * w, len);
+	}
+}
+
+// CRC-32 over 16 bytes (words w[0..3]) with the slice-by-16 tables: byte j of
+// the 16 goes through table 15 - j, so only the first word waits for the
+// running value and the other 12 lookups are independent of it.
+__device__ __forceinline__ uint32_t crc_16bytes(uint32_t crc, const uint32_t* w, const uint32_t (*t)[256]) {
+	const uint32_t x = crc ^ w[0];
+	return t[15][x & 255] ^ t[14][(x >> 8) & 255] ^ t[13][(x >> 16) & 255] ^ t[12][x >> 24] ^ t[11][w[1] & 255] ^
+	       t[10][(w[1] >> 8) & 255] ^ t[9][(w[1] >> 16) & 255] ^ t[8][w[1] >> 24] ^ t[7][w[2] & 255] ^
+	       t[6][(w[2] >> 8) & 255] ^ t[5][(w[2] >> 16) & 255] ^ t[4][w[2] >> 24] ^ t[3][w[3] & 255] ^
+	       t[2][(w[3] >> 8) & 255] ^ t[1][(w[3] >> 16) & 255] ^ t[0][w[3] >> 24];
+}
+
+__device__ __forceinline__ uint32_t crc_4bytes(uint32_t crc, uint32_t w, const uint32_t (*t)[256]) {
+	const uint32_t x = crc ^ w;
+	return t[3][x & 255] ^ t[2][(x >> 8) & 255] ^ t[1][(x >> 16) & 255] ^ t[0][x >> 24];
+}
+
+// a * b mod P for CRC-32 register values (reflected: bit 31 is the x^0
+// coefficient, so multiplying by x is a right shift with the reflected
+// polynomial folded back in).  Multiplying a register by x^(8n) mod P is
+// feeding it n zero bytes.
+__host__ __device__ inline uint32_t crc_mulmod(uint32_t a, uint32_t b) {
+	uint32_t p = 0;
+	for(int i = 31; i >= 0; i--) {
+		if((a >> i) & 1u) p ^= b;
+		b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+	}
+	return p;
+}
+
+// MD5 of slot k's bytes [0, lens[k]) followed by zeros up to slice_size, on
+// one lane.  The chain is serial, so a file with few large slices is bound by
+// it, not by bandwidth; the next block's loads are issued before the current
+// block is hashed so that their latency hides behind it.
+__device__ __forceinline__ void slice_md5_lane(const uint8_t* __restrict__ slots, size_t slot_bytes, unsigned k,
+                                               const uint32_t* __restrict__ lens, size_t slice_size,
+                                               uint8_t* __restrict__ md5_out) {
 	const uint32_t* src = reinterpret_cast<const uint32_t*>(slots + (size_t)k * slot_bytes);
 	size_t len = lens ? lens[k] : slice_size;
 	if(len > slice_size) len = slice_size;
 	uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
-	uint32_t crc = 0xFFFFFFFFu;
 	const size_t full = slice_size / 64;
-	uint32_t m[16];
+	uint32_t m[16], nx[16];
+	if(full) load_block(src, 0, len, nx);
 	for(size_t blk = 0; blk < full; blk++) {
-		const size_t base = blk * 64;
-		if(base + 64 <= len) {
 #pragma unroll
-			for(int w = 0; w < 16; w += 4) {
-				const uint4 v = *reinterpret_cast<const uint4*>(src + base / 4 + w);
-				m[w] = v.x; m[w + 1] = v.y; m[w + 2] = v.z; m[w + 3] = v.w;
-			}
-		} else if(base >= len) {
-#pragma unroll
-			for(int w = 0; w < 16; w++) m[w] = 0;
-		} else {
-#pragma unroll
-			for(int w = 0; w < 16; w++) m[w] = src[base / 4 + w] & keep4(base + 4 * w, len);
-		}
-#pragma unroll
-		for(int w = 0; w < 16; w++) {
-			const uint32_t x = crc ^ m[w];
-			crc = s_crc[3][x & 255] ^ s_crc[2][(x >> 8) & 255] ^ s_crc[1][(x >> 16) & 255] ^ s_crc[0][x >> 24];
-		}
+		for(int w = 0; w < 16; w++) m[w] = nx[w];
+		if(blk + 1 < full) load_block(src, (blk + 1) * 64, len, nx);
 		md5_block(st, m);
 	}
-	// tail (slice_size % 64 bytes, a multiple of 4 for PAR2 slices) + MD5 padding
+	// tail (slice_size % 64 bytes, a multiple of 4) + MD5 padding
 	const size_t rem = slice_size - full * 64;
 	const size_t base = full * 64;
 #pragma unroll
 	for(int w = 0; w < 16; w++) {
 		const size_t pos = base + 4 * w;
-		uint32_t v = 0;
-		if(pos < slice_size) {
-			v = pos < len ? src[pos / 4] & keep4(pos, len) : 0u;
-			const unsigned nb = (unsigned)(slice_size - pos < 4 ? slice_size - pos : 4);
-			v &= nb == 4 ? 0xFFFFFFFFu : ((1u << (8 * nb)) - 1u);
-			// CRC over the remaining bytes of the slice, byte by byte
-			for(unsigned bb = 0; bb < nb; bb++) crc = s_crc[0][(crc ^ (v >> (8 * bb))) & 255] ^ (crc >> 8);
-		}
-		m[w] = v;
+		m[w] = pos < slice_size && pos < len ? src[pos / 4] & keep4(pos, len) : 0u;
 	}
 	// append 0x80 after the message
 	m[rem / 4] |= 0x80u << (8 * (rem & 3));
@@ -192,7 +213,88 @@ __global__ __launch_bounds__(64) void k_slice_hash(const uint8_t* __restrict__ s
 	o[1] = st[1];
 	o[2] = st[2];
 	o[3] = st[3];
-	crc_out[k] = crc ^ 0xFFFFFFFFu;
+}
+
+// How the CRC lanes split a slice of S bytes over L = 2^lanes_log2 lanes.
+// The slice is read as lead = L*C - S zero bytes followed by the slice, in L
+// chunks of C bytes: leading zeros leave a zero register at zero, so the
+// register (started at zero) of that message is the slice's.  Lane j takes
+// chunk j; pairs of neighbouring results combine as left * x^(8 * right
+// length) + right, and every right-hand part at tree level l is 2^l whole
+// chunks, so the factors are per launch.  The CRC's initial value
+// 0xFFFFFFFF, carried through the S bytes, is added at the end.
+struct CrcSplit {
+	uint64_t chunk;       // C, a multiple of 16
+	uint64_t lead;        // L * C - S
+	uint32_t lanes_log2;  // L <= 64: a slice's lanes are in one wavefront
+	uint32_t init;        // 0xFFFFFFFF * x^(8 S) mod P
+	uint32_t x[6];        // x^(8 C 2^l) mod P for tree level l
+};
+
+// CRC-32 of slot k's bytes [0, lens[k]) followed by zeros up to slice_size,
+// on L lanes per slice (slice-by-16 tables in LDS), combined in registers;
+// `lane` numbers the (slice, chunk) pairs.  Every lane of the wavefront
+// calls it: the combine shuffles read neighbouring lanes.
+__device__ __forceinline__ void slice_crc_lanes(const uint8_t* __restrict__ slots, size_t slot_bytes, unsigned count,
+                                                const uint32_t* __restrict__ lens, size_t slice_size,
+                                                uint32_t* __restrict__ crc_out, const uint32_t (*s_crc)[256],
+                                                const CrcSplit& sp, size_t lane) {
+	const size_t k = lane >> sp.lanes_log2;
+	const size_t j = lane & ((1u << sp.lanes_log2) - 1);
+	uint32_t r = 0;
+	if(k < count) {
+		const uint32_t* w = reinterpret_cast<const uint32_t*>(slots + k * slot_bytes);
+		size_t len = lens ? lens[k] : slice_size;
+		if(len > slice_size) len = slice_size;
+		const size_t c0 = j * sp.chunk, c1 = c0 + sp.chunk;  // chunk j of the lead-padded message
+		size_t p = c0 > sp.lead ? c0 - sp.lead : 0;
+		const size_t end = c1 > sp.lead ? (c1 - sp.lead < slice_size ? c1 - sp.lead : slice_size) : 0;
+		if(p < len) {  // a chunk that starts past the data is all zeros: its register stays zero
+			// words up to 64-byte alignment, 64-byte blocks with the next one's loads
+			// in flight, then the remaining words
+			for(; p < end && (p & 63); p += 4) r = crc_4bytes(r, w[p / 4] & keep4(p, len), s_crc);
+			if(p + 64 <= end) {
+				uint32_t m[16], nx[16];
+				load_block(w, p, len, nx);
+				for(; p + 64 <= end; p += 64) {
+#pragma unroll
+					for(int q = 0; q < 16; q++) m[q] = nx[q];
+					if(p + 128 <= end) load_block(w, p + 64, len, nx);
+#pragma unroll
+					for(int q = 0; q < 16; q += 4) r = crc_16bytes(r, m + q, s_crc);
+				}
+			}
+			for(; p < end; p += 4) r = crc_4bytes(r, w[p / 4] & keep4(p, len), s_crc);
+		}
+	}
+	// every lane takes part in the shuffles; slice groups never straddle a wavefront
+	for(unsigned l = 0; l < sp.lanes_log2; l++) {
+		const uint32_t right = __shfl_down(r, 1u << l, 64);
+		if(!(j & ((2u << l) - 1))) r = crc_mulmod(r, sp.x[l]) ^ right;
+	}
+	if(k < count && j == 0) crc_out[k] = ~(r ^ sp.init);
+}
+
+// Per-slice MD5 and CRC-32 in one launch: the first md5_blocks blocks run the
+// MD5 chains (64 slices each, on the block's first wavefront, so that the
+// chains spread over the CUs), the rest the CRC lanes, which fill the SIMDs
+// the chains leave idle.
+__global__ __launch_bounds__(256) void k_slice_hash(const uint8_t* __restrict__ slots, size_t slot_bytes,
+                                                    unsigned count, const uint32_t* __restrict__ lens,
+                                                    size_t slice_size, uint8_t* __restrict__ md5_out,
+                                                    uint32_t* __restrict__ crc_out,
+                                                    const uint32_t* __restrict__ crc_tab, CrcSplit sp,
+                                                    unsigned md5_blocks) {
+	__shared__ uint32_t s_crc[16][256];
+	if(blockIdx.x < md5_blocks) {
+		const unsigned k = blockIdx.x * 64 + threadIdx.x;
+		if(threadIdx.x < 64 && k < count) slice_md5_lane(slots, slot_bytes, k, lens, slice_size, md5_out);
+		return;
+	}
+	for(unsigned i = threadIdx.x; i < 16 * 256; i += 256) s_crc[i >> 8][i & 255] = crc_tab[i];
+	__syncthreads();
+	slice_crc_lanes(slots, slot_bytes, count, lens, slice_size, crc_out, s_crc, sp,
+	                (size_t)(blockIdx.x - md5_blocks) * 256 + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ per file
@@ -300,13 +402,41 @@ __global__ __launch_bounds__(64) void k_file_md5_round(const Md5Span* __restrict
 	md5_span(e);
 }
 
+namespace {
+// x^(8n) mod P by square and multiply (x^1 is 0x40000000 in the reflected form)
+uint32_t crc_xpow8n(uint64_t n) {
+	uint32_t r = 0x80000000u, sq = 0x40000000u;
+	for(uint64_t e = 8 * n; e; e >>= 1) {
+		if(e & 1) r = crc_mulmod(r, sq);
+		sq = crc_mulmod(sq, sq);
+	}
+	return r;
+}
+
+CrcSplit crc_split(size_t slice_size) {
+	// about 4 KiB or more per lane, at most one wavefront per slice
+	unsigned lg = 0;
+	while(lg < 6 && (slice_size >> (lg + 1)) >= 4096) lg++;
+	CrcSplit sp{};
+	const size_t L = (size_t)1 << lg;
+	sp.lanes_log2 = lg;
+	sp.chunk = ((slice_size + L - 1) / L + 15) / 16 * 16;
+	sp.lead = L * sp.chunk - slice_size;
+	sp.init = crc_mulmod(0xFFFFFFFFu, crc_xpow8n(slice_size));
+	for(unsigned l = 0; l < lg; l++) sp.x[l] = crc_xpow8n(sp.chunk << l);
+	return sp;
+}
+}  // namespace
+
 hipError_t launch_slice_hash(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
                              size_t slice_size, uint8_t* md5_out, uint32_t* crc_out, const DeviceTables& t,
                              hipStream_t s) {
 	if(!count) return hipSuccess;
 	if(slice_size % 4 || slice_size > slot_bytes || slot_bytes % 16 || !t.crc) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(k_slice_hash, dim3((count + 63) / 64), dim3(64), 0, s, slots, slot_bytes, count, lens, slice_size,
-	                   md5_out, crc_out, t.crc);
+	const CrcSplit sp = crc_split(slice_size);
+	const size_t md5_blocks = (count + 63) / 64, crc_blocks = (((size_t)count << sp.lanes_log2) + 255) / 256;
+	hipLaunchKernelGGL(k_slice_hash, dim3((unsigned)(md5_blocks + crc_blocks)), dim3(256), 0, s, slots, slot_bytes,
+	                   count, lens, slice_size, md5_out, crc_out, t.crc, sp, (unsigned)md5_blocks);
 	return hipGetLastError();
 }
 

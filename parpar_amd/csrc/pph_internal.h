@@ -39,7 +39,7 @@ struct DeviceTables {
 const uint16_t* host_exp();
 const uint16_t* host_log();
 const uint16_t* host_rowmask();  // 16 x 4 x 16 nibble tables of the coefficient bit-matrices
-const uint32_t* host_crc();      // CRC-32 (reflected 0xEDB88320) slice-by-4 tables, 4 x 256
+const uint32_t* host_crc();      // CRC-32 (reflected 0xEDB88320) slice-by-16 tables, 16 x 256
 uint16_t host_input_log(uint16_t input);
 uint16_t host_coeff_log(uint16_t input_log, uint16_t exponent);
 

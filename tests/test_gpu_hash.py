@@ -46,12 +46,31 @@ def slice_hash(datas, S):
     return md5.download(16 * n).reshape(n, 16), crc.download(4 * n).view(np.uint32)
 
 
-@pytest.mark.parametrize("S", [4, 60, 64, 100, 124, 128, 4096 + 52, 262144, 1048576 + 4])
+# slice sizes span every split of the CRC over lanes (1, 2, 4, 16 and 64 lanes
+# per slice, with and without a lead of virtual zero bytes)
+@pytest.mark.parametrize("S", [4, 60, 64, 100, 124, 128, 4096 + 52, 8192, 20000, 65536 + 12, 262144, 1048576 + 4])
 def test_slice_md5_crc(S):
     say("slice hash", S)
     rng = np.random.default_rng(S)
     lens = [S, 0, 1, S - 1, int(rng.integers(0, S + 1)), 55 if S > 55 else S, 56 if S > 56 else S, 64 if S > 64 else S]
     datas = [O.prng_bytes(31, S % 1000, k, ln) for k, ln in enumerate(lens)]
+    md5, crc = slice_hash(datas, S)
+    for k, d in enumerate(datas):
+        padded = np.zeros(S, np.uint8)
+        padded[:len(d)] = d
+        assert bytes(md5[k]).hex() == hashlib.md5(padded.tobytes()).hexdigest(), (S, k, len(d))
+        assert int(crc[k]) == zlib.crc32(padded.tobytes()) & 0xffffffff, (S, k, len(d))
+
+
+@pytest.mark.parametrize("S,n", [(8192, 300), (65536 + 12, 40)])
+def test_slice_hash_many(S, n):
+    """Many slices of ragged lengths: the CRC lanes of one slice and the
+    slices of one launch spread over several workgroups."""
+    say("slice hash many", S, n)
+    rng = np.random.default_rng(n)
+    lens = rng.integers(0, S + 1, n)
+    lens[::7] = S
+    datas = [O.prng_bytes(33, n, k, int(ln)) for k, ln in enumerate(lens)]
     md5, crc = slice_hash(datas, S)
     for k, d in enumerate(datas):
         padded = np.zeros(S, np.uint8)
