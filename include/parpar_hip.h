@@ -208,6 +208,11 @@ uint16_t pph_bilinear_mul(uint16_t c, uint16_t x);
  * data-form slots (16-byte aligned), before log-domain preparation. */
 int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens, size_t slice_size,
                    uint8_t* md5_out, uint32_t* crc_out, void* stream);
+/* The same hashes as IFSC entries (src/gf.cc:845-891, the input file slice
+ * checksums of the PAR2 IFSC packet): slice k's MD5 then its CRC-32 (little
+ * endian) at ifsc_out + 20k (device; 4-byte aligned). */
+int pph_slice_hash_ifsc(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                        size_t slice_size, uint8_t* ifsc_out, void* stream);
 /* MD5 of a file laid out over `count` slots, slice_size bytes each (the last
  * one partial): file_size bytes in total (hasher_base.h:178-186).  One serial
  * chain, run by one wavefront. */
@@ -273,6 +278,34 @@ int pph_batch_run(pph_batch* b, int use_graph, void* stream);
  * events recorded on its stream around them; waits for that run. */
 int pph_batch_fused_ms(pph_batch* b, float* ms);
 void pph_batch_destroy(pph_batch* b);
+
+/* ======================================================================
+ * Recovery-packet MD5 on the device: the MD5Multi of HasherOutput
+ * (src/gf.cc:1070-1219) as lib/par2.js:243-262,300-327 feeds it.  Region r
+ * hashes recovery packet r from its recovery set ID on (lib/par2.js:505-513):
+ * set_id (16 bytes) || "PAR 2.0\0RecvSlic" || u32le(exps[r]) || the recovery
+ * slice, which may arrive in pieces (chunked mode).  One lane per region.
+ * ====================================================================== */
+typedef struct pph_packet_md5 pph_packet_md5;
+/* State for `regions` packets (1..65535) on the current device. */
+int pph_packet_md5_create(unsigned regions, pph_packet_md5** out);
+void pph_packet_md5_destroy(pph_packet_md5* h);
+/* Start every region's message; set_id (16 bytes) and exps (regions values
+ * < 65535) are host arrays, read before the call returns. */
+int pph_packet_md5_begin(pph_packet_md5* h, const uint8_t* set_id, const uint16_t* exps, void* stream);
+/* Append bytes [0, len) of every region's slot: slot r at slots + r *
+ * slot_bytes (device, 16-byte aligned slot_bytes); len even and len + 4 <=
+ * slot_bytes; slots NULL appends len zero bytes. */
+int pph_packet_md5_update(pph_packet_md5* h, const void* slots, size_t slot_bytes, size_t len, void* stream);
+/* Pad and write the 16-byte digests to md5_out (device, 16 x regions);
+ * pph_packet_md5_begin starts the next packets. */
+int pph_packet_md5_final(pph_packet_md5* h, uint8_t* md5_out, void* stream);
+/* Backend: append every recovery slice's current bytes (the current slice
+ * size, after the batches submitted so far; zeros when nothing was added) to
+ * h, which has one region per recovery slice, and return once appended.  The
+ * recovery packets' MD5 once the last chunk is in, without copying the slices
+ * to the host first. */
+int pph_proc_packet_md5_update(pph_proc* p, pph_packet_md5* h);
 
 /* ======================================================================
  * Serving: jobs arrive over time and run with continuous batching

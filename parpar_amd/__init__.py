@@ -103,6 +103,13 @@ _SIGS = {
     "pph_batch_fused_ms": (_i, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "pph_slice_hash": (_i, [_vp, _sz, _u, _vp, _sz, _vp, _vp, _vp]),
     "pph_file_md5": (_i, [_vp, _sz, _u, _sz, _u64, _vp, _vp]),
+    "pph_slice_hash_ifsc": (_i, [_vp, _sz, _u, _vp, _sz, _vp, _vp]),
+    "pph_packet_md5_create": (_i, [_u, _vp]),
+    "pph_packet_md5_destroy": (None, [_vp]),
+    "pph_packet_md5_begin": (_i, [_vp, _vp, _vp, _vp]),
+    "pph_packet_md5_update": (_i, [_vp, _vp, _sz, _sz, _vp]),
+    "pph_packet_md5_final": (_i, [_vp, _vp, _vp]),
+    "pph_proc_packet_md5_update": (_i, [_vp, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
     "pph_muladd_bilinear": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
     "pph_bilinear_workspace": (_i, [_sz, _u, _u, ctypes.POINTER(_sz)]),

@@ -365,6 +365,95 @@ extern "C" int pph_slice_hash(const void* slots, size_t slot_bytes, unsigned cou
 	return PPH_OK;
 }
 
+extern "C" int pph_slice_hash_ifsc(const void* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                                   size_t slice_size, uint8_t* ifsc_out, void* stream) {
+	if((!slots || !ifsc_out) && count) return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	hipError_t e = launch_slice_hash_to((const uint8_t*)slots, slot_bytes, count, lens, slice_size, ifsc_out, 20,
+	                                    ifsc_out + 16, 20, *t, (hipStream_t)stream);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	return PPH_OK;
+}
+
+// ---------------------------------------------------------------- recovery-packet MD5
+
+struct pph_packet_md5 {
+	int device = 0;
+	unsigned regions = 0;
+	uint32_t* d_state = nullptr;  // 20 words per region
+	uint16_t* d_exps = nullptr;
+	uint32_t head[8] = {};
+	uint64_t done = 0;  // message bytes of each region so far
+	bool begun = false, first = false;
+};
+
+extern "C" int pph_packet_md5_create(unsigned regions, pph_packet_md5** out) {
+	if(!out || !regions || regions > 65535) return PPH_E_INVALID;
+	*out = nullptr;
+	auto* h = new(std::nothrow) pph_packet_md5;
+	if(!h) return PPH_E_NOMEM;
+	h->regions = regions;
+	if(hipGetDevice(&h->device) != hipSuccess || hipMalloc((void**)&h->d_state, (size_t)regions * 80) != hipSuccess ||
+	   hipMalloc((void**)&h->d_exps, (size_t)regions * 2) != hipSuccess) {
+		(void)hipGetLastError();
+		pph_packet_md5_destroy(h);
+		return PPH_E_NOMEM;
+	}
+	*out = h;
+	return PPH_OK;
+}
+
+extern "C" void pph_packet_md5_destroy(pph_packet_md5* h) {
+	if(!h) return;
+	if(h->d_state) (void)hipFree(h->d_state);
+	if(h->d_exps) (void)hipFree(h->d_exps);
+	delete h;
+}
+
+extern "C" int pph_packet_md5_begin(pph_packet_md5* h, const uint8_t* set_id, const uint16_t* exps, void* stream) {
+	if(!h || !set_id || !exps) return PPH_E_INVALID;
+	for(unsigned r = 0; r < h->regions; r++)
+		if(exps[r] == 65535) return PPH_E_INVALID;
+	// the host array is read before the call returns
+	PPH_HIP(hipMemcpyAsync(h->d_exps, exps, (size_t)h->regions * 2, hipMemcpyHostToDevice, (hipStream_t)stream));
+	PPH_HIP(hipStreamSynchronize((hipStream_t)stream));
+	static const char kType[16] = {'P', 'A', 'R', ' ', '2', '.', '0', '\0', 'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+	memcpy(h->head, set_id, 16);
+	memcpy(h->head + 4, kType, 16);
+	h->done = 36;
+	h->begun = h->first = true;
+	return PPH_OK;
+}
+
+namespace {
+int packet_md5_step(pph_packet_md5* h, const uint8_t* slots, size_t slot_bytes, size_t len, uint8_t* digest,
+                    hipStream_t s) {
+	if(!h->begun) return PPH_E_STATE;
+	hipError_t e = launch_packet_md5(h->d_state, slots, slot_bytes, h->regions, h->d_exps, h->head,
+	                                 h->first ? 0 : h->done, len, h->first, digest != nullptr, digest, s);
+	if(e == hipErrorInvalidValue) return PPH_E_INVALID;
+	if(e != hipSuccess) return set_error(e);
+	h->done += len;
+	h->first = false;
+	if(digest) h->begun = false;
+	return PPH_OK;
+}
+}  // namespace
+
+extern "C" int pph_packet_md5_update(pph_packet_md5* h, const void* slots, size_t slot_bytes, size_t len,
+                                     void* stream) {
+	if(!h) return PPH_E_INVALID;
+	return packet_md5_step(h, (const uint8_t*)slots, slot_bytes, len, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int pph_packet_md5_final(pph_packet_md5* h, uint8_t* md5_out, void* stream) {
+	if(!h || !md5_out) return PPH_E_INVALID;
+	return packet_md5_step(h, nullptr, 0, 0, md5_out, (hipStream_t)stream);
+}
+
 extern "C" int pph_file_md5(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                             uint64_t file_size, uint8_t* md5_out, void* stream) {
 	if(!slots || !md5_out) return PPH_E_INVALID;
@@ -932,6 +1021,19 @@ extern "C" int pph_proc_get_output(pph_proc* p, unsigned index, void* output, in
 		}
 	}
 	if(valid) *valid = p->h_valid[index];
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_packet_md5_update(pph_proc* p, pph_packet_md5* h) {
+	if(!p || !h) return PPH_E_INVALID;
+	if(!p->inited) return PPH_E_STATE;
+	if(h->regions != p->exps.size() || h->device != p->device) return PPH_E_INVALID;
+	// after every submitted batch (same stream); nothing computed reads as zeros.  The
+	// backend's stream does not order with the caller's, so the call returns once done.
+	const int rc =
+	    packet_md5_step(h, p->processing_add ? p->d_out : nullptr, p->slot_bytes, p->cur_size, nullptr, p->s_comp);
+	if(rc) return rc;
+	PPH_HIP(hipStreamSynchronize(p->s_comp));
 	return PPH_OK;
 }
 

@@ -169,13 +169,23 @@ __host__ __device__ inline uint32_t crc_mulmod(uint32_t a, uint32_t b) {
 	return p;
 }
 
+// Where slice k's hashes go: MD5 (16 bytes) at md5 + k * md5_stride, CRC-32
+// (4 bytes, little endian) at crc + k * crc_stride.  Strides 16 and 4 give two
+// arrays; 20 and 20 with crc = md5 + 16 give IFSC entries.
+struct SliceHashOut {
+	uint8_t* md5;
+	size_t md5_stride;
+	uint8_t* crc;
+	size_t crc_stride;
+};
+
 // MD5 of slot k's bytes [0, lens[k]) followed by zeros up to slice_size, on
 // one lane.  The chain is serial, so a file with few large slices is bound by
 // it, not by bandwidth; the next block's loads are issued before the current
 // block is hashed so that their latency hides behind it.
 __device__ __forceinline__ void slice_md5_lane(const uint8_t* __restrict__ slots, size_t slot_bytes, unsigned k,
                                                const uint32_t* __restrict__ lens, size_t slice_size,
-                                               uint8_t* __restrict__ md5_out) {
+                                               const SliceHashOut& out) {
 	const uint32_t* src = reinterpret_cast<const uint32_t*>(slots + (size_t)k * slot_bytes);
 	size_t len = lens ? lens[k] : slice_size;
 	if(len > slice_size) len = slice_size;
@@ -208,7 +218,7 @@ __device__ __forceinline__ void slice_md5_lane(const uint8_t* __restrict__ slots
 	m[14] = (uint32_t)bits;
 	m[15] = (uint32_t)(bits >> 32);
 	md5_block(st, m);
-	uint32_t* o = reinterpret_cast<uint32_t*>(md5_out + 16 * (size_t)k);
+	uint32_t* o = reinterpret_cast<uint32_t*>(out.md5 + out.md5_stride * k);
 	o[0] = st[0];
 	o[1] = st[1];
 	o[2] = st[2];
@@ -237,7 +247,7 @@ struct CrcSplit {
 // calls it: the combine shuffles read neighbouring lanes.
 __device__ __forceinline__ void slice_crc_lanes(const uint8_t* __restrict__ slots, size_t slot_bytes, unsigned count,
                                                 const uint32_t* __restrict__ lens, size_t slice_size,
-                                                uint32_t* __restrict__ crc_out, const uint32_t (*s_crc)[256],
+                                                const SliceHashOut& out, const uint32_t (*s_crc)[256],
                                                 const CrcSplit& sp, size_t lane) {
 	const size_t k = lane >> sp.lanes_log2;
 	const size_t j = lane & ((1u << sp.lanes_log2) - 1);
@@ -272,7 +282,7 @@ __device__ __forceinline__ void slice_crc_lanes(const uint8_t* __restrict__ slot
 		const uint32_t right = __shfl_down(r, 1u << l, 64);
 		if(!(j & ((2u << l) - 1))) r = crc_mulmod(r, sp.x[l]) ^ right;
 	}
-	if(k < count && j == 0) crc_out[k] = ~(r ^ sp.init);
+	if(k < count && j == 0) *reinterpret_cast<uint32_t*>(out.crc + out.crc_stride * k) = ~(r ^ sp.init);
 }
 
 // Per-slice MD5 and CRC-32 in one launch: the first md5_blocks blocks run the
@@ -281,20 +291,150 @@ __device__ __forceinline__ void slice_crc_lanes(const uint8_t* __restrict__ slot
 // the chains leave idle.
 __global__ __launch_bounds__(256) void k_slice_hash(const uint8_t* __restrict__ slots, size_t slot_bytes,
                                                     unsigned count, const uint32_t* __restrict__ lens,
-                                                    size_t slice_size, uint8_t* __restrict__ md5_out,
-                                                    uint32_t* __restrict__ crc_out,
+                                                    size_t slice_size, SliceHashOut out,
                                                     const uint32_t* __restrict__ crc_tab, CrcSplit sp,
                                                     unsigned md5_blocks) {
 	__shared__ uint32_t s_crc[16][256];
 	if(blockIdx.x < md5_blocks) {
 		const unsigned k = blockIdx.x * 64 + threadIdx.x;
-		if(threadIdx.x < 64 && k < count) slice_md5_lane(slots, slot_bytes, k, lens, slice_size, md5_out);
+		if(threadIdx.x < 64 && k < count) slice_md5_lane(slots, slot_bytes, k, lens, slice_size, out);
 		return;
 	}
 	for(unsigned i = threadIdx.x; i < 16 * 256; i += 256) s_crc[i >> 8][i & 255] = crc_tab[i];
 	__syncthreads();
-	slice_crc_lanes(slots, slot_bytes, count, lens, slice_size, crc_out, s_crc, sp,
+	slice_crc_lanes(slots, slot_bytes, count, lens, slice_size, out, s_crc, sp,
 	                (size_t)(blockIdx.x - md5_blocks) * 256 + threadIdx.x);
+}
+
+// ------------------------------------------------------------------ recovery packets
+
+// One call of the recovery-packet hash.  Region r's message is the packet from
+// its recovery set ID on (lib/par2.js:505-513): set ID (16 bytes) ||
+// "PAR 2.0\0RecvSlic" || u32le(exps[r]) || the recovery slice, which may come
+// in pieces of even length (chunked mode, lib/par2.js:300-327).  Every region
+// has the same length, so the fill of the partial block is uniform.
+struct PacketMd5Call {
+	uint32_t head[8];  // set ID, then the packet type, as little-endian words
+	uint64_t done;     // message bytes of every region before this call (ignored with kPacketFirst)
+	uint64_t n;        // data bytes this call appends to every region (even)
+	unsigned flags;
+};
+constexpr unsigned kPacketFirst = 1, kPacketFinal = 2;
+
+// Data bytes [o, o + 4) as a little-endian word (o even), zero past n (n even).
+__device__ __forceinline__ uint32_t packet_word(const uint32_t* __restrict__ d, size_t o, size_t n) {
+	if(!d || o >= n) return 0u;
+	const uint32_t v = (o & 3) ? __builtin_amdgcn_alignbyte(d[(o + 2) / 4], d[(o - 2) / 4], 2) : d[o / 4];
+	return o + 2 == n ? v & 0xFFFFu : v;
+}
+
+// `blocks` whole 64-byte blocks from data offset o (SHIFT: o = 2 mod 4, each
+// word is the high half of one data word and the low half of the next), the
+// next block's loads in flight while the current one is hashed.
+template <bool SHIFT>
+__device__ __forceinline__ void packet_blocks(uint32_t st[4], const uint32_t* __restrict__ d, size_t o, size_t blocks) {
+	const uint32_t* p = d + (SHIFT ? (o + 2) / 4 : o / 4);
+	uint32_t carry = SHIFT ? d[(o - 2) / 4] : 0u;
+	uint32_t nx[16], m[16];
+#pragma unroll
+	for(int i = 0; i < 16; i++) nx[i] = p[i];
+	for(size_t b = 0; b < blocks; b++) {
+#pragma unroll
+		for(int i = 0; i < 16; i++) m[i] = SHIFT ? __builtin_amdgcn_alignbyte(nx[i], i ? nx[i - 1] : carry, 2) : nx[i];
+		if(SHIFT) carry = nx[15];
+		if(b + 1 < blocks) {
+#pragma unroll
+			for(int i = 0; i < 16; i++) nx[i] = p[16 * (b + 1) + i];
+		}
+		md5_block(st, m);
+	}
+}
+
+// Lane r: region r's MD5 state (4 words) and partial block (16 words, zero
+// past the fill) live in state[20 r ...] between calls.  The partial block is
+// completed from the data, whole blocks are hashed straight from the slot,
+// and the rest becomes the new partial block; with kPacketFinal the message
+// is padded (RFC 1321) and the digest written.  Slots NULL: the data is zeros.
+__global__ __launch_bounds__(64) void k_packet_md5(uint32_t* __restrict__ state, const uint8_t* __restrict__ slots,
+                                                   size_t slot_bytes, unsigned count,
+                                                   const uint16_t* __restrict__ exps, PacketMd5Call c,
+                                                   uint8_t* __restrict__ digest) {
+	const unsigned r = blockIdx.x * 64 + threadIdx.x;
+	if(r >= count) return;
+	uint32_t* sv = state + (size_t)r * 20;
+	uint32_t st[4], buf[16];
+	uint64_t done = c.done;
+	if(c.flags & kPacketFirst) {
+		st[0] = 0x67452301u;
+		st[1] = 0xefcdab89u;
+		st[2] = 0x98badcfeu;
+		st[3] = 0x10325476u;
+#pragma unroll
+		for(int k = 0; k < 16; k++) buf[k] = k < 8 ? c.head[k] : 0u;
+		buf[8] = exps[r];
+		done = 36;
+	} else {
+#pragma unroll
+		for(int k = 0; k < 4; k++) st[k] = sv[k];
+#pragma unroll
+		for(int k = 0; k < 16; k++) buf[k] = sv[4 + k];
+	}
+	const uint32_t* d = slots ? reinterpret_cast<const uint32_t*>(slots + (size_t)r * slot_bytes) : nullptr;
+	const size_t n = c.n;
+	const unsigned f = (unsigned)(done & 63);  // bytes in the partial block
+#pragma unroll
+	for(int k = 0; k < 16; k++) {
+		const int delta = 4 * k - (int)f;  // data offset of word k
+		if(delta == -2) {
+			if(d && n >= 2) buf[k] |= d[0] << 16;
+		} else if(delta >= 0) {
+			buf[k] = packet_word(d, (size_t)delta, n);
+		}
+	}
+	if(f + n >= 64) {
+		md5_block(st, buf);
+		size_t o = 64 - f;
+		const size_t blocks = (n - o) / 64;
+		if(blocks) {
+			if(!d) {
+				uint32_t z[16];
+#pragma unroll
+				for(int k = 0; k < 16; k++) z[k] = 0u;
+				for(size_t b = 0; b < blocks; b++) md5_block(st, z);
+			} else if(o & 3) {
+				packet_blocks<true>(st, d, o, blocks);
+			} else {
+				packet_blocks<false>(st, d, o, blocks);
+			}
+		}
+		o += 64 * blocks;
+#pragma unroll
+		for(int k = 0; k < 16; k++) buf[k] = packet_word(d, o + 4 * k, n);
+	}
+	done += n;
+	if(c.flags & kPacketFinal) {
+		const unsigned fill = (unsigned)(done & 63);
+#pragma unroll
+		for(int k = 0; k < 16; k++)
+			if(k == (int)(fill / 4)) buf[k] |= 0x80u << (8 * (fill & 3));
+		if(fill >= 56) {
+			md5_block(st, buf);
+#pragma unroll
+			for(int k = 0; k < 16; k++) buf[k] = 0u;
+		}
+		const uint64_t bits = done * 8;
+		buf[14] = (uint32_t)bits;
+		buf[15] = (uint32_t)(bits >> 32);
+		md5_block(st, buf);
+		uint32_t* o = reinterpret_cast<uint32_t*>(digest + 16 * (size_t)r);
+#pragma unroll
+		for(int k = 0; k < 4; k++) o[k] = st[k];
+	} else {
+#pragma unroll
+		for(int k = 0; k < 4; k++) sv[k] = st[k];
+#pragma unroll
+		for(int k = 0; k < 16; k++) sv[4 + k] = buf[k];
+	}
 }
 
 // ------------------------------------------------------------------ per file
@@ -428,16 +568,25 @@ CrcSplit crc_split(size_t slice_size) {
 }
 }  // namespace
 
+hipError_t launch_slice_hash_to(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                                size_t slice_size, uint8_t* md5, size_t md5_stride, uint8_t* crc, size_t crc_stride,
+                                const DeviceTables& t, hipStream_t s) {
+	if(!count) return hipSuccess;
+	if(slice_size % 4 || slice_size > slot_bytes || slot_bytes % 16 || !t.crc) return hipErrorInvalidValue;
+	if(md5_stride % 4 || crc_stride % 4 || (size_t)md5 % 4 || (size_t)crc % 4) return hipErrorInvalidValue;
+	const CrcSplit sp = crc_split(slice_size);
+	const SliceHashOut out{md5, md5_stride, crc, crc_stride};
+	const size_t md5_blocks = (count + 63) / 64, crc_blocks = (((size_t)count << sp.lanes_log2) + 255) / 256;
+	hipLaunchKernelGGL(k_slice_hash, dim3((unsigned)(md5_blocks + crc_blocks)), dim3(256), 0, s, slots, slot_bytes,
+	                   count, lens, slice_size, out, t.crc, sp, (unsigned)md5_blocks);
+	return hipGetLastError();
+}
+
 hipError_t launch_slice_hash(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
                              size_t slice_size, uint8_t* md5_out, uint32_t* crc_out, const DeviceTables& t,
                              hipStream_t s) {
-	if(!count) return hipSuccess;
-	if(slice_size % 4 || slice_size > slot_bytes || slot_bytes % 16 || !t.crc) return hipErrorInvalidValue;
-	const CrcSplit sp = crc_split(slice_size);
-	const size_t md5_blocks = (count + 63) / 64, crc_blocks = (((size_t)count << sp.lanes_log2) + 255) / 256;
-	hipLaunchKernelGGL(k_slice_hash, dim3((unsigned)(md5_blocks + crc_blocks)), dim3(256), 0, s, slots, slot_bytes,
-	                   count, lens, slice_size, md5_out, crc_out, t.crc, sp, (unsigned)md5_blocks);
-	return hipGetLastError();
+	return launch_slice_hash_to(slots, slot_bytes, count, lens, slice_size, md5_out, 16,
+	                            reinterpret_cast<uint8_t*>(crc_out), 4, t, s);
 }
 
 hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
@@ -461,6 +610,21 @@ hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned cou
 hipError_t launch_file_md5_round(const void* spans, unsigned n, hipStream_t s) {
 	if(!n) return hipSuccess;
 	hipLaunchKernelGGL(k_file_md5_round, dim3(n), dim3(64), 0, s, reinterpret_cast<const Md5Span*>(spans));
+	return hipGetLastError();
+}
+
+hipError_t launch_packet_md5(uint32_t* state, const uint8_t* slots, size_t slot_bytes, unsigned count,
+                             const uint16_t* exps, const uint32_t head[8], uint64_t done, uint64_t n, bool first,
+                             bool final, uint8_t* digest, hipStream_t s) {
+	if(!count) return hipSuccess;
+	if(n % 2 || (slots && (slot_bytes % 16 || n + 4 > slot_bytes))) return hipErrorInvalidValue;
+	PacketMd5Call c{};
+	for(int k = 0; k < 8; k++) c.head[k] = head[k];
+	c.done = done;
+	c.n = n;
+	c.flags = (first ? kPacketFirst : 0u) | (final ? kPacketFinal : 0u);
+	hipLaunchKernelGGL(k_packet_md5, dim3((count + 63) / 64), dim3(64), 0, s, state, slots, slot_bytes, count, exps, c,
+	                   digest);
 	return hipGetLastError();
 }
 

@@ -64,9 +64,20 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
 hipError_t launch_muladd_dense(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
                                const uint16_t* exps, uint16_t* out, unsigned n_out, bool accumulate,
                                const DeviceTables& t, hipStream_t s);
+// md5 / crc strides in bytes (16 and 4: two arrays; 20 and 20 with crc = md5 + 16: IFSC entries)
+hipError_t launch_slice_hash_to(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
+                                size_t slice_size, uint8_t* md5, size_t md5_stride, uint8_t* crc, size_t crc_stride,
+                                const DeviceTables& t, hipStream_t s);
 hipError_t launch_slice_hash(const uint8_t* slots, size_t slot_bytes, unsigned count, const uint32_t* lens,
                              size_t slice_size, uint8_t* md5_out, uint32_t* crc_out, const DeviceTables& t,
                              hipStream_t s);
+// Recovery-packet MD5 over `count` regions (state: 20 words each, device).
+// first: start with head (set ID, packet type) and u32 exps[r]; done: message
+// bytes before this call; n: data bytes of each slot appended (even; slots
+// NULL appends zeros); final: pad and write 16-byte digests.
+hipError_t launch_packet_md5(uint32_t* state, const uint8_t* slots, size_t slot_bytes, unsigned count,
+                             const uint16_t* exps, const uint32_t head[8], uint64_t done, uint64_t n, bool first,
+                             bool final, uint8_t* digest, hipStream_t s);
 hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                            uint64_t file_size, uint8_t* md5_out, hipStream_t s);
 // Spans of file MD5 chains (n Md5Span records, device-resident), one wave each.

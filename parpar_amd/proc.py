@@ -171,6 +171,53 @@ class PAR2ProcHIP:
         check(lib().pph_proc_debug_corrupt_output(self._p, index, offset, mask), "debug_corrupt_output")
 
 
+class PacketMd5:
+    """Recovery-packet MD5 on the device: HasherOutput's MD5Multi
+    (src/gf.cc:1070-1219) fed as lib/par2.js:243-262,300-327 feeds it.  Region
+    r hashes set_id || "PAR 2.0\\0RecvSlic" || u32le(exps[r]) || the recovery
+    slice, appended in pieces of even length."""
+
+    def __init__(self, regions):
+        h = ctypes.c_void_p()
+        check(lib().pph_packet_md5_create(int(regions), ctypes.byref(h)), "pph_packet_md5_create")
+        self._h = h
+        self.regions = int(regions)
+        self._out = DeviceBuffer(16 * self.regions)
+
+    def begin(self, set_id, exps):
+        sid = np.frombuffer(bytes(set_id), np.uint8)
+        assert sid.size == 16
+        e = np.ascontiguousarray(exps, dtype=np.uint16)
+        assert e.size == self.regions
+        check(lib().pph_packet_md5_begin(self._h, _ptr(sid), _ptr(e), None), "pph_packet_md5_begin")
+
+    def update(self, slots, slot_bytes, length):
+        """Append bytes [0, length) of each slot (DeviceBuffer, or None: zeros)."""
+        ptr = slots.ptr if slots is not None else None
+        check(lib().pph_packet_md5_update(self._h, ptr, slot_bytes, length, None), "pph_packet_md5_update")
+
+    def update_from(self, proc):
+        """Append the backend's current recovery slices."""
+        check(lib().pph_proc_packet_md5_update(proc._p, self._h), "pph_proc_packet_md5_update")
+
+    def final(self):
+        check(lib().pph_packet_md5_final(self._h, self._out.ptr, None), "pph_packet_md5_final")
+        check(lib().pph_synchronize(), "pph_synchronize")
+        return self._out.download(16 * self.regions).reshape(self.regions, 16)
+
+    def close(self):
+        if self._h is not None:
+            lib().pph_packet_md5_destroy(self._h)
+            self._h = None
+        self._out.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PAR2Proc:
     """ParPar's controller over backends that each own a byte range of every slice."""
 
