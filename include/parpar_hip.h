@@ -337,8 +337,9 @@ typedef struct pph_serve_config {
  * MD5 and CRC-32 of the slice zero-padded to slice_size (the IFSC entries,
  * hasher/hasher_base.h:154-176), the MD5 of the file (hasher_base.h:178-186),
  * and n_out recovery slices with their gf16_cksum flags (PAR2Proc over the
- * file's slices, gf16/controller.h:248-326).  Output pointers are host memory,
- * may be NULL, and are written before the job's result is posted. */
+ * file's slices, gf16/controller.h:248-326), and on request each recovery
+ * packet's MD5.  Output pointers are host memory, may be NULL, and are
+ * written before the job's result is posted. */
 typedef struct pph_serve_job {
 	uint64_t file_size;         /* >= 1 */
 	size_t slice_size;          /* multiple of 4; ceil(file_size / slice_size) + first_input <= 32768 */
@@ -354,6 +355,10 @@ typedef struct pph_serve_job {
 	uint8_t* slice_md5;         /* 16 bytes per slice */
 	uint32_t* slice_crc;        /* one per slice */
 	uint8_t* file_md5;          /* 16 bytes */
+	uint8_t* ifsc;              /* 20 bytes per slice: MD5 then CRC-32, the IFSC entries (NULL: none) */
+	const uint8_t* packet_set_id;  /* 16 bytes, copied at submit: the recovery set ID; with it, each
+	                                  recovery packet's MD5 (lib/par2.js:505-513) goes to packet_md5 */
+	uint8_t* packet_md5;        /* 16 bytes per recovery slice */
 } pph_serve_job;
 
 typedef struct pph_serve_result {

@@ -108,9 +108,13 @@ struct Job {
 	// arena ranges: A = inputs + hash results, B = compute buffers
 	size_t off_a = 0, bytes_a = 0, off_b = 0, bytes_b = 0;
 	bool have_a = false, have_b = false;
-	uint8_t *data = nullptr, *smd5 = nullptr, *digest = nullptr;
-	uint32_t *lens = nullptr, *scrc = nullptr, *md5_state = nullptr;
+	uint8_t *data = nullptr, *ifsc = nullptr, *digest = nullptr;  // ifsc: slice MD5 + CRC-32, 20 bytes each
+	uint32_t *lens = nullptr, *md5_state = nullptr;
 	uint8_t *logd = nullptr, *out = nullptr;
+	uint32_t* pk_state = nullptr;  // recovery-packet MD5 state (20 words per recovery slice)
+	uint8_t* pk_md5 = nullptr;
+	bool packet = false;           // recovery-packet MD5 requested
+	uint32_t packet_head[8] = {};  // set ID and packet type words
 	uint16_t *logs = nullptr, *d_exps = nullptr;
 	int* valid = nullptr;
 	int stream = -1;
@@ -188,11 +192,12 @@ struct pph_serve {
 namespace {
 
 size_t bytes_a(size_t n_in, size_t slot) {
-	return round_up(n_in * slot, 256) + 2 * round_up(n_in * 4, 256) + round_up(n_in * 16, 256) + 2 * 256;
+	return round_up(n_in * slot, 256) + round_up(n_in * 4, 256) + round_up(n_in * 20, 256) + 2 * 256;
 }
-size_t bytes_b(size_t n_in, unsigned n_out, size_t slot) {
+size_t bytes_b(size_t n_in, unsigned n_out, size_t slot, bool packet) {
 	return round_up(n_in * slot, 256) + round_up((size_t)n_out * slot, 256) + round_up(n_in * 2, 256) +
-	       round_up((size_t)n_out * 2, 256) + round_up((size_t)n_out * 4, 256);
+	       round_up((size_t)n_out * 2, 256) + round_up((size_t)n_out * 4, 256) +
+	       (packet ? round_up((size_t)n_out * 80, 256) + round_up((size_t)n_out * 16, 256) : 0);
 }
 
 void carve(pph_serve* s, Job* j) {
@@ -204,8 +209,7 @@ void carve(pph_serve* s, Job* j) {
 	};
 	j->data = take(j->n_in * j->slot);
 	j->lens = (uint32_t*)take(j->n_in * 4);
-	j->scrc = (uint32_t*)take(j->n_in * 4);
-	j->smd5 = take(j->n_in * 16);
+	j->ifsc = take(j->n_in * 20);
 	j->md5_state = (uint32_t*)take(16);
 	j->digest = take(16);
 	p = s->arena_mem + j->off_b;
@@ -214,6 +218,10 @@ void carve(pph_serve* s, Job* j) {
 	j->logs = (uint16_t*)take(j->n_in * 2);
 	j->d_exps = (uint16_t*)take((size_t)j->n_out * 2);
 	j->valid = (int*)take((size_t)j->n_out * 4);
+	if(j->packet) {
+		j->pk_state = (uint32_t*)take((size_t)j->n_out * 80);
+		j->pk_md5 = take((size_t)j->n_out * 16);
+	}
 }
 
 // Hashes and recovery of one job on its stream, after its inputs.
@@ -222,7 +230,8 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 	const size_t S = j->spec.slice_size;
 	hipError_t e = hipEventRecord(j->ev_data, a);
 	if(e == hipSuccess)
-		e = launch_slice_hash(j->data, j->slot, (unsigned)j->n_in, j->lens, S, j->smd5, j->scrc, *s->tab, a);
+		e = launch_slice_hash_to(j->data, j->slot, (unsigned)j->n_in, j->lens, S, j->ifsc, 20, j->ifsc + 16, 20,
+		                         *s->tab, a);
 	if(e == hipSuccess && j->n_out && j->form == PPH_FORM_BILINEAR &&
 	   bilinear_prepare(j->slot, (unsigned)j->n_in, j->n_out) != PPH_OK)
 		j->form = PPH_FORM_PLAIN;
@@ -244,6 +253,10 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 			                        nullptr, 0, (uint16_t*)j->out, j->n_out, false, 0, *s->tab, a);
 		}
 		if(e == hipSuccess) e = launch_verify(j->out, j->slot, j->n_out, S, ck_offset(S), s->ck, j->valid, *s->tab, a);
+		// recovery packets hashed while the slices are still in HBM (lib/par2.js:505-513)
+		if(e == hipSuccess && j->packet)
+			e = launch_packet_md5(j->pk_state, j->out, j->slot, j->n_out, j->d_exps, j->packet_head, 0, S, true, true,
+			                      j->pk_md5, a);
 	}
 	if(e == hipSuccess) e = hipEventRecord(j->ev_done, a);
 	return e;
@@ -343,8 +356,12 @@ void finish_compute(pph_serve* s, Job* j) {
 	const pph_serve_job& sp = j->spec;
 	hipError_t e = hipSuccess;
 	if(j->status == PPH_OK) {
-		if(sp.slice_md5) e = hipMemcpy(sp.slice_md5, j->smd5, j->n_in * 16, hipMemcpyDeviceToHost);
-		if(e == hipSuccess && sp.slice_crc) e = hipMemcpy(sp.slice_crc, j->scrc, j->n_in * 4, hipMemcpyDeviceToHost);
+		if(sp.slice_md5) e = hipMemcpy2D(sp.slice_md5, 16, j->ifsc, 20, 16, j->n_in, hipMemcpyDeviceToHost);
+		if(e == hipSuccess && sp.slice_crc)
+			e = hipMemcpy2D(sp.slice_crc, 4, j->ifsc + 16, 20, 4, j->n_in, hipMemcpyDeviceToHost);
+		if(e == hipSuccess && sp.ifsc) e = hipMemcpy(sp.ifsc, j->ifsc, j->n_in * 20, hipMemcpyDeviceToHost);
+		if(e == hipSuccess && j->packet && sp.packet_md5)
+			e = hipMemcpy(sp.packet_md5, j->pk_md5, (size_t)j->n_out * 16, hipMemcpyDeviceToHost);
 		if(e == hipSuccess && sp.valid && j->n_out)
 			e = hipMemcpy(sp.valid, j->valid, (size_t)j->n_out * 4, hipMemcpyDeviceToHost);
 		if(e == hipSuccess && sp.recovery_out && j->n_out)
@@ -824,8 +841,15 @@ extern "C" int pph_serve_submit(pph_serve* s, const pph_serve_job* job, uint64_t
 		   bilinear_workspace_bytes(j->slot, (unsigned)j->n_in, j->n_out) <= s->cfg.bilinear_workspace)
 			j->form = PPH_FORM_BILINEAR;
 	}
+	if(job->packet_set_id && j->n_out) {
+		static const char kType[16] = {'P', 'A', 'R', ' ', '2', '.', '0', '\0', 'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+		j->packet = true;
+		memcpy(j->packet_head, job->packet_set_id, 16);
+		memcpy(j->packet_head + 4, kType, 16);
+	}
+	j->spec.packet_set_id = nullptr;  // copied: the caller's buffer may go
 	j->bytes_a = bytes_a(j->n_in, j->slot);
-	j->bytes_b = bytes_b(j->n_in, j->n_out, j->slot);
+	j->bytes_b = bytes_b(j->n_in, j->n_out, j->slot, j->packet);
 	if(j->bytes_a + j->bytes_b > s->cfg.hbm_budget) {
 		delete j;
 		return PPH_E_NOMEM;  // could never be admitted

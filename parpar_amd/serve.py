@@ -25,7 +25,8 @@ class ServeJobDesc(ctypes.Structure):
                 ("first_input", ctypes.c_uint), ("exps", ctypes.c_void_p), ("host_data", ctypes.c_void_p),
                 ("seed", ctypes.c_uint64), ("job_id", ctypes.c_uint32), ("recovery_out", ctypes.c_void_p),
                 ("valid", ctypes.c_void_p), ("slice_md5", ctypes.c_void_p), ("slice_crc", ctypes.c_void_p),
-                ("file_md5", ctypes.c_void_p)]
+                ("file_md5", ctypes.c_void_p), ("ifsc", ctypes.c_void_p), ("packet_set_id", ctypes.c_void_p),
+                ("packet_md5", ctypes.c_void_p)]
 
 
 class ServeResult(ctypes.Structure):
@@ -54,10 +55,12 @@ class FileJob:
     data: the file bytes (host) or None for device-generated inputs (slice k =
     the counter-based generator's (seed, job_id, k)).  keep_recovery=False
     leaves the recovery slices on the device (only their flags come back).
+    keep_ifsc: also the slice hashes as IFSC entries (20 bytes per slice).
+    packet_set_id (16 bytes): also each recovery packet's MD5.
     """
 
     def __init__(self, file_size, slice_size, n_out, exps=None, data=None, seed=0, job_id=0, first_input=0,
-                 keep_recovery=True, keep_slice_hashes=True):
+                 keep_recovery=True, keep_slice_hashes=True, keep_ifsc=False, packet_set_id=None):
         self.file_size = int(file_size)
         self.slice_size = int(slice_size)
         self.n_in = -(-self.file_size // self.slice_size)
@@ -72,13 +75,19 @@ class FileJob:
         self.slice_md5 = np.zeros((self.n_in, 16), np.uint8) if keep_slice_hashes else None
         self.slice_crc = np.zeros(self.n_in, np.uint32) if keep_slice_hashes else None
         self.file_md5 = np.zeros(16, np.uint8)
+        self.ifsc = np.zeros((self.n_in, 20), np.uint8) if keep_ifsc else None
+        self.packet_set_id = None if packet_set_id is None else np.frombuffer(bytes(packet_set_id), np.uint8).copy()
+        if self.packet_set_id is not None:
+            assert self.packet_set_id.size == 16
+        self.packet_md5 = np.zeros((max(self.n_out, 1), 16), np.uint8) if packet_set_id is not None else None
         self.result = None
 
     def desc(self):
         p = lambda a: None if a is None else _ptr(a)
         return ServeJobDesc(self.file_size, self.slice_size, self.n_out, self.first_input, p(self.exps),
                             p(self.data), self.seed, self.job_id, p(self.recovery), p(self.valid),
-                            p(self.slice_md5), p(self.slice_crc), p(self.file_md5))
+                            p(self.slice_md5), p(self.slice_crc), p(self.file_md5), p(self.ifsc),
+                            p(self.packet_set_id), p(self.packet_md5))
 
     @property
     def mul_adds(self):

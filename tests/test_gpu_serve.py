@@ -92,6 +92,37 @@ def test_serve_mixed_jobs():
         check_job(job)
 
 
+def test_serve_ifsc_and_packet_md5():
+    """IFSC entries and recovery-packet MD5s from the engine: against hashlib /
+    zlib over the padded slices and over the recovery packets assembled as
+    lib/par2.js:505-513 writes them, for host and device inputs, both forms."""
+    import struct
+    set_id = bytes(range(100, 116))
+    say("serve ifsc + packet md5")
+    for tuning in (None, B.fixed_tuning(plain=0.001, dense=0.0, bilinear=1.0)):
+        jobs = []
+        for j, (fs, S, R, host) in enumerate([(5 * 65536 + 1001, 65536, 3, True), (40 * 4096, 4096, 7, False),
+                                               (999, 1024, 2, True), (17 * 8192 - 6, 8192, 5, False)]):
+            data = O.prng_bytes(7, 2000 + j, 0, fs) if host else None
+            jobs.append(SV.FileJob(fs, S, R, data=data, seed=6, job_id=j, first_input=j, keep_ifsc=True,
+                                   packet_set_id=set_id))
+        kw = {} if tuning is None else dict(bilinear_workspace=4 << 30, tuning=tuning)
+        with SV.Server(hbm_budget=128 * MiB, max_compute=3, **kw) as srv:
+            srv.run_all(jobs)
+        for job in jobs:
+            check_job(job)
+            data = file_bytes(job)
+            S = job.slice_size
+            padded = np.zeros(job.n_in * S, np.uint8)
+            padded[:job.file_size] = data
+            for k in range(job.n_in):
+                row = padded[k * S:(k + 1) * S].tobytes()
+                assert bytes(job.ifsc[k]) == hashlib.md5(row).digest() + struct.pack("<I", zlib.crc32(row)), k
+            for r in range(job.n_out):
+                pkt = set_id + b"PAR 2.0\0RecvSlic" + struct.pack("<I", r) + job.recovery[r].tobytes()
+                assert bytes(job.packet_md5[r]) == hashlib.md5(pkt).digest(), r
+
+
 def test_serve_alone_vs_mixed():
     """A job's outputs do not depend on what runs beside it."""
     def target():
