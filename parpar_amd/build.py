@@ -90,10 +90,29 @@ def build_test_drivers(verbose=False):
             print("built", exe)
 
 
+NODE_INCLUDE = "/usr/include/node"
+NODE_ADDON = os.path.join(LIBDIR, "gfproc_hip.node")
+
+
+def build_node_addon(verbose=False):
+    """The Node binding (GfProc over PAR2ProcHIP), where Node's headers are installed."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = NODE_ADDON + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I" + NODE_INCLUDE, "-I" + INCLUDE,
+          os.path.join(HERE, "node", "gfproc_hip.cc"), "-o", tmp, "-L" + LIBDIR, "-lparpar_hip",
+          "-Wl,-rpath,$ORIGIN", "-pthread"])
+    os.replace(tmp, NODE_ADDON)
+    if verbose:
+        print("built", NODE_ADDON)
+    return NODE_ADDON
+
+
 def build_all(verbose=False):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found")
     build_library(verbose)
+    build_node_addon(verbose)
     build_oracle(verbose)
     build_test_drivers(verbose)
 
