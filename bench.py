@@ -370,6 +370,74 @@ def be_batch(args):
     return args.batch if args.batch else 64
 
 
+def main_hashes(args):
+    """Per-slice MD5 + CRC-32 (pph_slice_hash) and recovery-packet MD5
+    (pph_packet_md5_*) on slices generated in HBM, timed with HIP events:
+    few large slices (bound by one MD5 chain per slice) to many small ones."""
+    import numpy as np
+    import parpar_amd
+    from parpar_amd import proc as P
+
+    parpar_amd.require_device()
+    L = parpar_amd.lib()
+    chk = parpar_amd.check
+    stream = ctypes.c_void_p()
+    chk(L.pph_stream_create(ctypes.byref(stream)))
+    t = ctypes.c_void_p()
+    chk(L.pph_timer_create(ctypes.byref(t)))
+    reps = max(1, args.steps or 3)
+
+    def timed(fn):
+        fn()
+        chk(L.pph_stream_synchronize(stream))
+        best = None
+        for _ in range(reps):
+            chk(L.pph_timer_start(t, stream))
+            fn()
+            chk(L.pph_timer_stop(t, stream))
+            chk(L.pph_stream_synchronize(stream))
+            ms = ctypes.c_float()
+            chk(L.pph_timer_elapsed_ms(t, ctypes.byref(ms)))
+            best = ms.value if best is None else min(best, ms.value)
+        return best
+
+    rows = []
+    for count, S in [(64, 16 << 20), (100, 16 << 20), (1024, 1 << 20), (32768, 65536)]:
+        _, slot = P.slot_layout(S, 64)
+        buf = P.DeviceBuffer(slot * count)
+        md5, crc = P.DeviceBuffer(16 * count), P.DeviceBuffer(4 * count)
+        chk(L.pph_fill_random(buf.ptr, slot, count, S, 7, 0, 0, stream))
+        ms = timed(lambda: chk(L.pph_slice_hash(buf.ptr, slot, count, None, S, md5.ptr, crc.ptr, stream)))
+        rows.append({"kernel": "k_slice_hash", "slices": count, "slice_bytes": S, "ms": round(ms, 3),
+                     "per_chain_MBps": round(S / ms / 1e3, 1), "GBps": round(count * S / ms / 1e6, 1)})
+        log("slice hash %5d x %8d: %.2f ms" % (count, S, ms))
+        del buf, md5, crc
+    # recovery packets: 200 slices of 1 MiB (the default workload's recovery count and slice size)
+    R, S = 200, 1 << 20
+    _, slot = P.slot_layout(S, 64)
+    buf = P.DeviceBuffer(slot * R)
+    chk(L.pph_fill_random(buf.ptr, slot, R, S, 8, 0, 0, stream))
+    h = ctypes.c_void_p()
+    chk(L.pph_packet_md5_create(R, ctypes.byref(h)))
+    dig = P.DeviceBuffer(16 * R)
+    sid = np.arange(16, dtype=np.uint8)
+    exps = np.arange(R, dtype=np.uint16)
+
+    def packets():
+        chk(L.pph_packet_md5_begin(h, sid.ctypes.data, exps.ctypes.data, stream))
+        chk(L.pph_packet_md5_update(h, buf.ptr, slot, S, stream))
+        chk(L.pph_packet_md5_final(h, dig.ptr, stream))
+
+    ms = timed(packets)
+    rows.append({"kernel": "k_packet_md5", "slices": R, "slice_bytes": S, "ms": round(ms, 3),
+                 "per_chain_MBps": round(S / ms / 1e3, 1), "GBps": round(R * S / ms / 1e6, 1)})
+    log("packet md5 %d x %d: %.2f ms" % (R, S, ms))
+    L.pph_packet_md5_destroy(h)
+    print(json.dumps({"metric": "hash kernels: best of %d launches" % reps, "unit": "ms", "n_gpus": 1,
+                      "data": "synthetic (counter-based random bytes generated on the device)",
+                      "rows": rows}))
+
+
 def main_batched(args):
     import numpy as np
     import parpar_amd
@@ -499,7 +567,7 @@ def parse():
     ap.add_argument("--slice", type=int, default=None)
     ap.add_argument("--recovery", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving", "backend"], default=None)
+    ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving", "backend", "hashes"], default=None)
     ap.add_argument("--batch", type=int, default=0, help="backend: input batch size (0 = the backend's default)")
     ap.add_argument("--window", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
@@ -606,6 +674,8 @@ def main():
         return main_serving(args)
     if args.form == "backend":
         return main_backend(args)
+    if args.form == "hashes":
+        return main_hashes(args)
 
     import numpy as np
     import parpar_amd
