@@ -22,7 +22,7 @@ def say(*a):
     print(*a, flush=True)
 
 
-FORMS = {"plain": parpar_amd.FORM_PLAIN, "bilinear": parpar_amd.FORM_BILINEAR}
+FORMS = {"plain": parpar_amd.FORM_PLAIN, "bilinear": parpar_amd.FORM_BILINEAR, "auto": parpar_amd.FORM_AUTO}
 
 
 def new_backend(slice_size, batch, areas=2, ck=64, form="plain"):
@@ -191,6 +191,37 @@ def test_fill_and_dummy_inputs(form):
     o = np.zeros(S, np.uint8)
     for r in range(2):
         assert be.getOutput(r, o) and (o == ref[r]).all()
+    be.close()
+
+
+def test_auto_form_large_batches():
+    """PPH_FORM_AUTO: batches of 512 inputs with 64 recovery slices take the
+    bilinear form, 8 recovery slices the plain one; both give the oracle's
+    bytes.  Slots that fillInput prepared for one form refuse a recovery count
+    that would switch to the other."""
+    S, n = 8192, 1100
+    data = np.stack([O.prng_bytes(25, 0, k, S) for k in range(n)])
+    for R in (64, 8):
+        say("auto form", R)
+        be = new_backend(S, 512, areas=2, form="auto")
+        exps = np.arange(3, 3 + R, dtype=np.uint16)
+        be.setRecoverySlices(exps)
+        for k in range(n):
+            be.addInput(data[k], S if k < n - 1 else 3000, inputNum=k)
+        be.endInput()
+        lens = [S] * (n - 1) + [3000]
+        padded = data.copy()
+        padded[-1, 3000:] = 0
+        ref = O.recover(padded, lens, S, exps)
+        o = np.zeros(S, np.uint8)
+        for r in range(R):
+            assert be.getOutput(r, o) and (o == ref[r]).all(), (R, r)
+        be.close()
+    be = new_backend(S, 512, areas=1, form="auto")
+    be.setRecoverySlices(np.arange(64, dtype=np.uint16))
+    be.fillInput(data[0])
+    with pytest.raises(parpar_amd.PPHError):
+        be.setRecoverySlices(np.arange(8, dtype=np.uint16))
     be.close()
 
 
