@@ -2,8 +2,10 @@
 benchmark workloads: 2000 inputs x 1 MiB with 200 recovery slices (plain
 form) and 32768 inputs x 64 KiB with 8192 recovery slices (bilinear form).
 
-- Every recovery slice's gf16_cksum checks (the checksum block is carried
-  through the same linear map as the data).
+- Every recovery slice's gf16_cksum checks against its data, and every
+  checksum block equals the oracle's recovery run over the inputs' checksum
+  blocks as if they were slices (the checksum is linear: ck(sum c_i x_i) =
+  sum c_i ck(x_i)), so all R rows are pinned to the CPU, not a sample.
 - Encode -> erase -> decode: erase k inputs, take k recovery slices, remove
   the surviving inputs' share (the plain kernel with explicit coefficients,
   accumulating), invert the k x k coefficient matrix over GF(2^16) and apply
@@ -76,7 +78,7 @@ def say(*a):
 def roundtrip(n, S, R, form, erased, rows, seed):
     L = parpar_amd.lib()
     chk = parpar_amd.check
-    _, slot = P.slot_layout(S, 64)
+    ck_off, slot = P.slot_layout(S, 64)
     logs = input_logs(n)
     assert logs[:n] == [L.pph_input_log(i) for i in range(n)]
     src, prepd, out = P.DeviceBuffer(slot * n), P.DeviceBuffer(slot * n), P.DeviceBuffer(slot * R)
@@ -96,6 +98,14 @@ def roundtrip(n, S, R, form, erased, rows, seed):
     flags = valid.download(4 * R).view(np.int32)
     assert (flags == 1).all(), np.flatnonzero(flags != 1)[:10]
     say(" encoded, every checksum valid:", n, S, R, form)
+    cks = np.empty((n, 64), np.uint8)
+    for i in range(n):
+        cks[i] = O.cksum(O.prng_bytes(seed, 0, i, S), S, 64)
+    want_ck = O.recover(cks, [64] * n, 64, np.arange(R, dtype=np.uint16))
+    got_ck = out.download(slot * R).reshape(R, slot)[:, ck_off:ck_off + 64]
+    bad = np.flatnonzero((got_ck != want_ck).any(axis=1))
+    assert bad.size == 0, bad[:10]
+    say(" every checksum block equals the oracle's (linearity)")
 
     k = len(erased)
     syn = P.DeviceBuffer(slot * k)
