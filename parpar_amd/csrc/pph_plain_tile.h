@@ -82,8 +82,10 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, const uint16_t
 				lc = coef[(size_t)i * coef_stride + r0 + r];
 				use = lc != kSkipCoef;
 			} else {
+				// rows past n_out have exponent 0: their sums are never stored, and
+				// without a branch per row the lookups of neighbouring rows overlap
 				lc = 2 * mod65535(lin * e[r]);
-				use = r0 + r < n_out;
+				use = true;
 			}
 			if(use) {
 #pragma unroll
