@@ -55,7 +55,10 @@ FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 MFMA
 # DRAM bytes per mul-add launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by a 1 GiB plain copy in the same runs
 # (profiles/k_muladd_plain_2000x1MiB_R200.md), keyed by (form, inputs, slice, recovery).
-MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 17.18e9 + 0.210e9}
+MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 17.18e9 + 0.210e9,
+                    # k_blm_gemm<256>, the dominant kernel of the bilinear call
+                    # (profiles/bilinear_32768x64KiB_R8192.md)
+                    ("bilinear", 32768, 1 << 16, 8192): 431.9e9 + 3.34e9}
 
 CONFIGS = {
     1: dict(inputs=2000, slice=1 << 20, recovery=200, seed=1, form="plain", steps=10, warmup=2),
