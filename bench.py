@@ -176,11 +176,15 @@ def main_serving(args):
     th = threading.Thread(target=submitter)
     th.start()
     results = []
+    records = []
     kept = {}
     last_print = t0
     while len(results) < len(arrivals):
         for job in srv.poll(wait_ms=500):
             results.append(job.result)
+            records.append(dict(job_id=job.job_id, file_size=job.file_size, slice_size=job.slice_size,
+                                n_in=job.n_in, n_out=job.n_out, **{k: round(job.result[k], 4) for k in (
+                                    "latency_s", "queue_s", "compute_s", "run_s")}))
             if job.job_id in keep:
                 kept[job.job_id] = job
             if job.result["status"] != 0:
@@ -200,6 +204,10 @@ def main_serving(args):
     comp = np.array([r["compute_s"] + r["queue_s"] for r in results])
     bad_flags = sum(int((kept[k].valid[:kept[k].n_out] != 1).sum()) for k in kept)
     log("done in %.1f s; latency p50 %.2f s p99 %.2f s" % (makespan, np.percentile(lat, 50), np.percentile(lat, 99)))
+    if args.dump_jobs:
+        with open(args.dump_jobs, "w") as f:
+            for r in records:
+                f.write(json.dumps(r) + "\n")
 
     cpu = None
     if not args.no_cpu:
@@ -585,6 +593,7 @@ def parse():
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="serving: GPU_MAX_HW_QUEUES for this process (>= max-compute + 1 gives every compute "
                          "stream and the hash stream a hardware queue of its own; 0 leaves the environment as is)")
+    ap.add_argument("--dump-jobs", default=None, help="serving: write one JSON line per job (sizes, latencies)")
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--cksum", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")

@@ -333,6 +333,10 @@ typedef struct pph_serve_config {
 	                               min(staging_bytes, 4 MiB); 0: every file MD5 on the device */
 	unsigned upload_threads;    /* threads copying host-sourced inputs to the device, each with its own two
 	                               pinned staging chunks (0 = 1, at most 16) */
+	unsigned md5_order;         /* which queued file a free host MD5 thread takes: 0 = highest response
+	                               ratio, (wait + expected hash time) / expected hash time, so short files
+	                               do not queue behind multi-second chains while long ones still age to
+	                               the front; 1 = submit order */
 } pph_serve_config;
 
 /* One file.  Its slices (slice_size bytes, the last one partial) are PAR2

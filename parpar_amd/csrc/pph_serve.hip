@@ -97,6 +97,7 @@ constexpr unsigned kLookBehind = 8;   // queued jobs considered behind the head
 constexpr unsigned kMaxSpans = 1024;  // files per hash round
 constexpr unsigned kMaxUploaders = 16;
 constexpr size_t kMd5ReadChunk = 4u << 20;  // pinned read-back chunk of a host MD5 thread
+constexpr double kHostMd5Rate = 0.8e9;       // bytes per second of one host MD5 chain, for the queue order
 
 struct Job {
 	pph_serve_job spec;
@@ -658,8 +659,32 @@ int hash_device_file(pph_serve* s, Job* j, Stager& st, HostMd5& h) {
 	return e == hipSuccess ? PPH_OK : set_error(e);
 }
 
+// The queued file a free host MD5 thread takes (engine mutex held): the
+// highest response ratio, (wait + hash time) / hash time with the wait counted
+// from submit, or the oldest with md5_order = 1.  In a window that offers more
+// file bytes than the threads hash, submit order leaves small files behind
+// multi-second chains of large ones.
+Job* next_host_md5(pph_serve* s) {
+	auto pick = s->host_md5.begin();
+	if(s->cfg.md5_order == 0) {
+		const double now = now_s();
+		double best = -1;
+		for(auto it = s->host_md5.begin(); it != s->host_md5.end(); ++it) {
+			const double t = (double)(*it)->spec.file_size / kHostMd5Rate + 1e-3;
+			const double ratio = (now - (*it)->t_arrival + t) / t;
+			if(ratio > best) {
+				best = ratio;
+				pick = it;
+			}
+		}
+	}
+	Job* j = *pick;
+	s->host_md5.erase(pick);
+	return j;
+}
+
 // File MD5 on host thread k: host-sourced files from submit on, files
-// generated in HBM once their inputs are there; oldest first.
+// generated in HBM once their inputs are there.
 void md5_worker(pph_serve* s, unsigned k) {
 	bool device_set = false;
 	for(;;) {
@@ -668,8 +693,7 @@ void md5_worker(pph_serve* s, unsigned k) {
 			std::unique_lock<std::mutex> lk(s->mu);
 			s->cv.wait(lk, [&] { return s->sched_done || !s->host_md5.empty(); });
 			if(s->host_md5.empty()) break;
-			j = s->host_md5.front();
-			s->host_md5.pop_front();
+			j = next_host_md5(s);
 		}
 		HostMd5 h;
 		int rc = PPH_OK;
@@ -742,7 +766,7 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	if(!c.staging_bytes) c.staging_bytes = 64u << 20;
 	if(!c.upload_threads) c.upload_threads = 1;
 	if(!c.hbm_budget || !c.max_compute || c.max_compute > 64 || c.md5_quantum % 64 ||
-	   c.upload_threads > kMaxUploaders || c.md5_host_threads > 256)
+	   c.upload_threads > kMaxUploaders || c.md5_host_threads > 256 || c.md5_order > 1)
 		return PPH_E_INVALID;
 	int n = 0;
 	if(hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PPH_E_UNSUPPORTED;

@@ -17,7 +17,8 @@ from .proc import _ptr
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_compute", ctypes.c_uint), ("md5_quantum", ctypes.c_size_t),
                 ("staging_bytes", ctypes.c_size_t), ("bilinear_workspace", ctypes.c_size_t),
-                ("tuning", ctypes.c_void_p), ("md5_host_threads", ctypes.c_uint), ("upload_threads", ctypes.c_uint)]
+                ("tuning", ctypes.c_void_p), ("md5_host_threads", ctypes.c_uint), ("upload_threads", ctypes.c_uint),
+                ("md5_order", ctypes.c_uint)]
 
 
 class ServeJobDesc(ctypes.Structure):
@@ -100,17 +101,19 @@ class Server:
     them."""
 
     def __init__(self, hbm_budget, max_compute=8, md5_quantum=0, staging_bytes=0, device=0, bilinear_workspace=0,
-                 tuning=None, md5_host_threads=0, upload_threads=0):
+                 tuning=None, md5_host_threads=0, upload_threads=0, md5_order=0):
         """bilinear_workspace > 0 lets large jobs use the bilinear form; tuning
         (parpar_amd.batch.PPHTuning) gives the rates that choose it, None
         measures them at create.  md5_host_threads > 0 computes every file MD5
         on that many host threads (device-generated files read back from
-        HBM); upload_threads copy host bytes to the device (0 = 1)."""
+        HBM); upload_threads copy host bytes to the device (0 = 1); md5_order
+        0 hashes the queued file with the highest response ratio first, 1 in
+        submit order."""
         self._tuning = tuning  # kept alive while the engine reads it
         self.cfg = ServeConfig(int(hbm_budget), int(max_compute), int(md5_quantum), int(staging_bytes),
                                int(bilinear_workspace),
                                ctypes.cast(ctypes.pointer(tuning), ctypes.c_void_p) if tuning is not None else None,
-                               int(md5_host_threads), int(upload_threads))
+                               int(md5_host_threads), int(upload_threads), int(md5_order))
         h = ctypes.c_void_p()
         check(lib().pph_serve_create(device, ctypes.byref(self.cfg), ctypes.byref(h)), "pph_serve_create")
         self.h = h

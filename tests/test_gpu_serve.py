@@ -223,6 +223,28 @@ def test_serve_host_md5_and_uploaders():
     check_job(host)
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_serve_host_md5_order(order):
+    """One host MD5 thread, busy with a first file while a large and then a
+    small file queue: by response ratio (md5_order 0) the small one is hashed
+    first, in submit order (1) after the large one.  Every output checks."""
+    say("host md5 order", order)
+    first, large, small = (SV.FileJob(fs, MiB, 0, data=O.prng_bytes(12, k, 0, fs), job_id=k)
+                           for k, fs in enumerate([160 * MiB, 320 * MiB, MiB + 100]))
+    done = []
+    with SV.Server(hbm_budget=2048 * MiB, max_compute=4, md5_host_threads=1, upload_threads=2,
+                   md5_order=order) as srv:
+        for job in (first, large, small):
+            srv.submit(job)
+        while len(done) < 3:
+            done += [j.job_id for j in srv.poll(wait_ms=1000)]
+    say(" completion order", done)
+    assert done.index(small.job_id) < done.index(large.job_id) if order == 0 else \
+        done.index(small.job_id) > done.index(large.job_id)
+    for job in (first, large, small):
+        check_job(job)
+
+
 @pytest.mark.parametrize("quantum", [64, 4096, 8 * MiB])
 def test_serve_md5_rounds(quantum):
     """Files of many sizes hash in shared rounds of `quantum` bytes per chain."""
