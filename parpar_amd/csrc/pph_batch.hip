@@ -72,12 +72,22 @@ unsigned tune_bucket(unsigned r) {
 	return kTuneBuckets - 1;
 }
 
+// Share of the bilinear form's padded work that is real: its GEMM pads the
+// inputs to a multiple of 256 and the outputs to its row tile (128, or 256
+// where that pads no more).
+double bilinear_fill(unsigned n_in, unsigned n_out) {
+	const size_t rt = round_up(n_out, 128) < round_up(n_out, 256) ? 128 : 256;
+	return (double)n_in / (double)round_up(n_in, 256) * ((double)n_out / (double)round_up(n_out, rt));
+}
+
 double bilinear_rate(const pph_tuning& tu, unsigned n_in, unsigned n_out) {
 	const unsigned bk = tune_bucket(n_out);
 	if(tu.version < 3 || !(tu.bilinear[bk] > 0) || !(tu.bilinear_wide[bk] > 0) || !n_in) return 0;
 	// log-linear in the input count between the two measured widths (512, 4096)
 	const double f = std::min(1.0, std::max(0.0, (std::log2((double)n_in) - 9.0) / 3.0));
-	return std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
+	const double v = std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
+	// scaled by the job's padding against that of the measured shapes
+	return v * bilinear_fill(n_in, n_out) / bilinear_fill(512, kTuneR[bk]);
 }
 
 double plain_rate(const pph_tuning& tu, unsigned n_out) {
