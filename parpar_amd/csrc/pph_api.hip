@@ -802,10 +802,10 @@ extern "C" int pph_proc_create(int device, unsigned staging_areas, pph_proc** ou
 		pph_proc_destroy(p);
 		return rc;
 	}
-	// helper threads for the staging copies (PPH_COPY_THREADS, default 2: with
-	// the caller's thread, three copies at once fill the host's copy bandwidth
-	// into pinned memory; more threads contend and slow every copy)
-	unsigned helpers = 2;
+	// helper threads for the staging copies (PPH_COPY_THREADS, default 3: with
+	// the caller's thread, four streaming copies at once fill the host's copy
+	// bandwidth into pinned memory; more threads contend and slow every copy)
+	unsigned helpers = 3;
 	if(const char* v = getenv("PPH_COPY_THREADS")) helpers = (unsigned)std::min(15l, std::max(0l, strtol(v, nullptr, 10)));
 	p->copier.reset(new(std::nothrow) CopyPool(helpers));
 	if(!p->copier) {

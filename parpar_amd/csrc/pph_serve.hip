@@ -28,6 +28,7 @@
 // its MD5 chain ends) and the log-domain inputs and recovery slices (released
 // as soon as the recovery and its checks are copied back).
 #include "pph_internal.h"
+#include "pph_copy_pool.h"
 #include "pph_md5_host.h"
 #include "../../include/parpar_hip.h"
 
@@ -336,7 +337,7 @@ void upload_job(pph_serve* s, Job* j, Stager& st) {
 			const size_t n = std::min(chunk, len - done);
 			e = hipEventSynchronize(st.ev[which]);  // the last copy out of this chunk has finished
 			if(e != hipSuccess) break;
-			memcpy(st.buf[which], src + k * S + done, n);
+			copy_stream(st.buf[which], src + k * S + done, n);  // streaming stores: the DMA reads it next
 			e = hipMemcpyAsync(j->data + k * j->slot + done, st.buf[which], n, hipMemcpyHostToDevice, a);
 			if(e == hipSuccess) e = hipEventRecord(st.ev[which], a);
 			which ^= 1u;
