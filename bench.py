@@ -330,20 +330,28 @@ def main_backend(args):
     be.init(args.cksum, args.batch, parpar_amd.FORM_PLAIN)
     be.setRecoverySlices(np.arange(R, dtype=np.uint16))
 
+    phases = np.zeros(3)  # seconds in the adds, endInput, the getOutputs
+
     def step():
+        t = [time.perf_counter()]
         be.discardOutput()
         for i in range(n):
             if be.canAdd() == parpar_amd.PPH_ADD_FULL:
                 be.waitForAdd()
             be.addInput(inputs[i], S, inputNum=i)
+        t.append(time.perf_counter())
         be.endInput()
+        t.append(time.perf_counter())
         ok = True
         for r in range(R):
             ok = be.getOutput(r, out[r]) and ok
+        t.append(time.perf_counter())
+        phases[:] += np.diff(t)
         return ok
 
     for _ in range(args.warmup):
         step()
+    phases[:] = 0
     times, valid = [], True
     for _ in range(args.steps):
         t0 = time.perf_counter()
@@ -352,7 +360,9 @@ def main_backend(args):
     be.close()
     ms = 1e3 * float(np.mean(times))
     rate = 2.0 * ma / (ms / 1e3) / 1e9
-    log("backend: %.1f ms per pass, %.1f GB/s mul-add-equivalent, flags %s" % (ms, rate, valid))
+    phase_ms = [round(1e3 * x / args.steps, 2) for x in phases]
+    log("backend: %.1f ms per pass (adds %.1f, endInput %.1f, getOutputs %.1f), %.1f GB/s mul-add-equivalent, "
+        "flags %s" % (ms, *phase_ms, rate, valid))
     # CPU check on a column window of every input and recovery slice
     win = min(4096, S)
     tc = time.perf_counter()
@@ -369,6 +379,7 @@ def main_backend(args):
         "config": {"workload": "backend pass: %d host inputs x %d B -> %d recovery slices (exp 0..%d), input batch "
                                "%d, addInput/endInput/getOutput with PCIe both ways" % (
                                    n, S, R, R - 1, be_batch(args)), "inputs": n, "slice_bytes": S, "recovery": R},
+        "phases_ms": {"add_inputs": phase_ms[0], "end_input": phase_ms[1], "get_outputs": phase_ms[2]},
         "cpu_check": {"window_bytes": win, "matches_gpu": ok, "mul_adds": n * R * (win // 2),
                       "seconds": round(t_cpu, 2)},
     }

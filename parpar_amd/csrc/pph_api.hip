@@ -523,8 +523,10 @@ struct pph_proc {
 	unsigned batches_started = 0;
 	bool processing_add = false;
 	std::unique_ptr<pph::CopyPool> copier;  // host copies into and out of pinned memory
-	uint8_t* h_bounce = nullptr;            // pinned, one slice: getOutput's device-to-host copy
+	uint8_t* h_bounce[2] = {nullptr, nullptr};  // pinned, one slice each: getOutput's device-to-host copies
 	size_t bounce_bytes = 0;
+	int pf_index = -1;     // recovery slice already copied (or in flight on s_comp) into h_bounce[pf_buf]
+	unsigned pf_buf = 0;
 };
 
 namespace {
@@ -571,6 +573,7 @@ void free_outputs(pph_proc* p) {
 	p->d_exps = nullptr;
 	p->d_valid = nullptr;
 	p->r_alloc = 0;
+	p->pf_index = -1;
 }
 
 // refresh isActive flags from the device
@@ -684,6 +687,7 @@ int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
 	p->active_count++;
 	p->batches_started++;
 	p->verified = false;
+	p->pf_index = -1;
 	return PPH_OK;
 }
 
@@ -826,7 +830,8 @@ extern "C" void pph_proc_destroy(pph_proc* p) {
 	if(p->ev_copied) (void)hipEventDestroy(p->ev_copied);
 	if(p->s_copy) (void)hipStreamDestroy(p->s_copy);
 	if(p->s_comp) (void)hipStreamDestroy(p->s_comp);
-	if(p->h_bounce) (void)hipHostFree(p->h_bounce);
+	for(uint8_t* b : p->h_bounce)
+		if(b) (void)hipHostFree(b);
 	delete p;
 }
 
@@ -858,6 +863,7 @@ extern "C" int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_ba
 	p->batches_started = 0;
 	p->processing_add = false;
 	p->verified = false;
+	p->pf_index = -1;
 	p->r_pad = 0;
 	p->inited = true;
 	return PPH_OK;
@@ -879,6 +885,7 @@ extern "C" int pph_proc_set_current_slice_size(pph_proc* p, size_t slice_size) {
 		p->slice_size = std::max(p->slice_size, slice_size);
 		p->processing_add = false;
 		p->verified = false;
+		p->pf_index = -1;
 		p->cur_area = p->cur_inputs = 0;
 		p->active_count = 0;
 		int rc = alloc_areas(p);
@@ -936,6 +943,7 @@ extern "C" int pph_proc_set_recovery_slices(pph_proc* p, unsigned n, const uint1
 	PPH_HIP(hipMemcpy(p->d_exps, ex.data(), r_pad * sizeof(uint16_t), hipMemcpyHostToDevice));
 	p->h_valid.assign(r_pad, 0);
 	p->verified = false;
+	p->pf_index = -1;
 	return PPH_OK;
 }
 
@@ -1029,24 +1037,39 @@ extern "C" int pph_proc_get_output(pph_proc* p, unsigned index, void* output, in
 		p->verified = true;
 	}
 	if(p->cur_size) {
-		// through a pinned bounce slice (full-rate DMA) and a parallel host copy
+		// through pinned bounce slices (full-rate DMA) and a parallel host copy;
+		// the next slice's device-to-host copy runs during this slice's host copy
 		if(p->bounce_bytes < p->cur_size) {
-			if(p->h_bounce) (void)hipHostFree(p->h_bounce);
-			p->h_bounce = nullptr;
+			for(uint8_t*& b : p->h_bounce) {
+				if(b) (void)hipHostFree(b);
+				b = nullptr;
+			}
 			p->bounce_bytes = 0;
-			if(hipHostMalloc((void**)&p->h_bounce, p->alloc_slice, hipHostMallocDefault) == hipSuccess)
-				p->bounce_bytes = p->alloc_slice;
-			else
-				p->h_bounce = nullptr;
+			p->pf_index = -1;
+			bool ok = true;
+			for(uint8_t*& b : p->h_bounce)
+				ok = ok && hipHostMalloc((void**)&b, p->alloc_slice, hipHostMallocDefault) == hipSuccess;
+			if(ok) p->bounce_bytes = p->alloc_slice;
 			(void)hipGetLastError();
 		}
-		const uint8_t* src = p->d_out + (size_t)index * p->slot_bytes;
-		if(p->h_bounce) {
-			PPH_HIP(hipMemcpyAsync(p->h_bounce, src, p->cur_size, hipMemcpyDeviceToHost, p->s_comp));
-			PPH_HIP(hipStreamSynchronize(p->s_comp));
-			p->copier->copy(output, p->h_bounce, p->cur_size);
+		const size_t n_out = p->exps.size();
+		if(p->bounce_bytes >= p->cur_size) {
+			const unsigned b = p->pf_buf;
+			if(p->pf_index != (int)index) {  // (a prefetched slice was waited for above)
+				PPH_HIP(hipMemcpyAsync(p->h_bounce[b], p->d_out + (size_t)index * p->slot_bytes, p->cur_size,
+				                       hipMemcpyDeviceToHost, p->s_comp));
+				PPH_HIP(hipStreamSynchronize(p->s_comp));
+			}
+			p->pf_index = -1;
+			if(index + 1 < n_out) {  // callers read the slices in order
+				PPH_HIP(hipMemcpyAsync(p->h_bounce[b ^ 1u], p->d_out + (size_t)(index + 1) * p->slot_bytes, p->cur_size,
+				                       hipMemcpyDeviceToHost, p->s_comp));
+				p->pf_index = (int)index + 1;
+				p->pf_buf = b ^ 1u;
+			}
+			p->copier->copy(output, p->h_bounce[b], p->cur_size);
 		} else {
-			PPH_HIP(hipMemcpy(output, src, p->cur_size, hipMemcpyDeviceToHost));
+			PPH_HIP(hipMemcpy(output, p->d_out + (size_t)index * p->slot_bytes, p->cur_size, hipMemcpyDeviceToHost));
 		}
 	}
 	if(valid) *valid = p->h_valid[index];
@@ -1097,5 +1120,6 @@ extern "C" int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t
 	PPH_HIP(launch_xor_byte(p->d_out + (size_t)index * p->slot_bytes + offset, mask, p->s_comp));
 	PPH_HIP(hipStreamSynchronize(p->s_comp));
 	p->verified = false;
+	p->pf_index = -1;
 	return PPH_OK;
 }

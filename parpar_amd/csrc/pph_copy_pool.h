@@ -6,6 +6,8 @@
 #include <emmintrin.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <cstring>
@@ -51,10 +53,13 @@ class CopyPool {
 		size_t n;
 	};
 	std::mutex mu_;
-	std::condition_variable cv_, done_cv_;
-	std::vector<Part> parts_;
-	size_t next_ = 0, left_ = 0;
-	bool stop_ = false;
+	std::condition_variable cv_;
+	std::vector<Part> parts_;      // the current copy's parts (mu_)
+	size_t next_ = 0;              // next part to take (mu_)
+	std::atomic<size_t> left_{0};  // parts not yet copied
+	std::atomic<unsigned> gen_{0};  // bumped each time parts are posted
+	std::atomic<bool> stop_{false};
+	static constexpr std::chrono::microseconds kSpin{200};
 	std::vector<std::thread> threads_;
 
 	bool take(Part* p) {  // with mu_ held
@@ -62,20 +67,33 @@ class CopyPool {
 		*p = parts_[next_++];
 		return true;
 	}
-	void finish() {  // with mu_ held
-		if(--left_ == 0) done_cv_.notify_all();
-	}
+	// copy_stream has fenced the part's stores: the release publishes them to
+	// the caller's acquire
+	void finish() { left_.fetch_sub(1, std::memory_order_release); }
 	void worker() {
+		unsigned seen = 0;
 		std::unique_lock<std::mutex> lk(mu_);
 		for(;;) {
-			cv_.wait(lk, [&] { return stop_ || next_ < parts_.size(); });
+			if(!stop_ && next_ >= parts_.size()) {
+				// Idle: watch for the next parts a while before sleeping.  Adds come
+				// back to back, and a wake-up from sleep takes longer than a part.
+				lk.unlock();
+				const auto t0 = std::chrono::steady_clock::now();
+				for(unsigned k = 1; gen_.load(std::memory_order_acquire) == seen && !stop_; k++) {
+					_mm_pause();
+					if(k % 64 == 0 && std::chrono::steady_clock::now() - t0 > kSpin) break;
+				}
+				lk.lock();
+				cv_.wait(lk, [&] { return stop_ || next_ < parts_.size(); });
+			}
 			if(stop_) return;
+			seen = gen_.load(std::memory_order_relaxed);
 			Part p;
 			while(take(&p)) {
 				lk.unlock();
 				copy_stream(p.dst, p.src, p.n);
-				lk.lock();
 				finish();
+				lk.lock();
 			}
 		}
 	}
@@ -105,21 +123,26 @@ public:
 			return;
 		}
 		const size_t step = (n / ways + 63) & ~size_t(63);
-		std::unique_lock<std::mutex> lk(mu_);
-		parts_.clear();
-		next_ = 0;
-		for(size_t off = 0; off < n; off += step)
-			parts_.push_back({(uint8_t*)dst + off, (const uint8_t*)src + off, std::min(step, n - off)});
-		left_ = parts_.size();
+		{
+			std::lock_guard<std::mutex> lk(mu_);
+			parts_.clear();
+			next_ = 0;
+			for(size_t off = 0; off < n; off += step)
+				parts_.push_back({(uint8_t*)dst + off, (const uint8_t*)src + off, std::min(step, n - off)});
+			left_.store(parts_.size(), std::memory_order_relaxed);
+			gen_.fetch_add(1, std::memory_order_release);
+		}
 		cv_.notify_all();
-		Part p;
-		while(take(&p)) {
-			lk.unlock();
+		for(Part p;;) {
+			{
+				std::lock_guard<std::mutex> lk(mu_);
+				if(!take(&p)) break;
+			}
 			copy_stream(p.dst, p.src, p.n);
-			lk.lock();
 			finish();
 		}
-		done_cv_.wait(lk, [&] { return left_ == 0; });
+		// the helpers' last parts are in progress: at most one part's copy time
+		while(left_.load(std::memory_order_acquire) != 0) _mm_pause();
 	}
 };
 
