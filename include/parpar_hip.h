@@ -65,6 +65,10 @@ int pph_last_hip_error(void);
 int pph_device_count(void);
 /* Device name and CU count of a device; name may be NULL. */
 int pph_device_info(int device, char* name, size_t name_len, int* cu_count);
+/* Device memory in bytes: the memory_global of ParPar's device info
+ * (src/gf.cc:777), which its planner sizes device shares by
+ * (lib/par2gen.js:509-516). */
+int pph_device_memory(int device, size_t* total_bytes);
 
 /* ---- PAR2 coefficient helpers (gf16/gfmat_coeff.c:56-77) ---- */
 uint16_t pph_input_log(uint16_t input_num);                     /* gfmat_input_log */

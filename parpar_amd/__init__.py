@@ -44,6 +44,7 @@ _SIGS = {
     "pph_last_hip_error": (_i, []),
     "pph_device_count": (_i, []),
     "pph_device_info": (_i, [_i, ctypes.c_char_p, _sz, ctypes.POINTER(_i)]),
+    "pph_device_memory": (_i, [_i, ctypes.POINTER(_sz)]),
     "pph_input_log": (_u16, [_u16]),
     "pph_coeff_log": (_u16, [_u16, _u16]),
     "pph_coeff": (_u16, [_u16, _u16]),

@@ -175,6 +175,14 @@ extern "C" int pph_device_info(int device, char* name, size_t name_len, int* cu_
 	return PPH_OK;
 }
 
+extern "C" int pph_device_memory(int device, size_t* total_bytes) {
+	if(!total_bytes) return PPH_E_INVALID;
+	hipDeviceProp_t prop;
+	PPH_HIP(hipGetDeviceProperties(&prop, device));
+	*total_bytes = prop.totalGlobalMem;
+	return PPH_OK;
+}
+
 extern "C" uint16_t pph_input_log(uint16_t input_num) { return host_input_log(input_num); }
 extern "C" uint16_t pph_coeff_log(uint16_t input_log, uint16_t exponent) {
 	return host_coeff_log(input_log, exponent);

@@ -719,6 +719,58 @@ napi_value Close(napi_env env, napi_callback_info info) {
 	return nullptr;
 }
 
+// hip_device_info(device) and hip_devices(): the fields of ParPar's OpenCL
+// device info (src/gf.cc:759-787) that its planner reads (lib/par2gen.js:
+// 498-516), for HIP devices.
+napi_value device_object(napi_env env, int device) {
+	char name[256] = {};
+	int cus = 0;
+	size_t mem = 0;
+	int rc = pph_device_info(device, name, sizeof(name), &cus);
+	if(!rc) rc = pph_device_memory(device, &mem);
+	if(rc) {
+		napi_throw_error(env, nullptr, ("Unable to obtain HIP device info: " + status_text(rc)).c_str());
+		return nullptr;
+	}
+	napi_value o, v;
+	NAPI_OK(napi_create_object(env, &o));
+	NAPI_OK(napi_create_int32(env, device, &v));
+	NAPI_OK(napi_set_named_property(env, o, "id", v));
+	NAPI_OK(napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &v));
+	NAPI_OK(napi_set_named_property(env, o, "name", v));
+	NAPI_OK(napi_create_double(env, (double)mem, &v));
+	NAPI_OK(napi_set_named_property(env, o, "memory_global", v));
+	NAPI_OK(napi_create_uint32(env, (uint32_t)cus, &v));
+	NAPI_OK(napi_set_named_property(env, o, "compute_units", v));
+	return o;
+}
+
+napi_value HipDeviceInfo(napi_env env, napi_callback_info info) {
+	size_t argc = 1;
+	napi_value argv[1];
+	NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+	int32_t device = 0;
+	if(argc >= 1 && !is_nullish(env, argv[0])) NAPI_OK(napi_get_value_int32(env, argv[0], &device));
+	const int n = pph_device_count();
+	if(device < 0 || device >= n) {
+		napi_throw_error(env, nullptr, "Invalid HIP device");
+		return nullptr;
+	}
+	return device_object(env, device);
+}
+
+napi_value HipDevices(napi_env env, napi_callback_info) {
+	const int n = std::max(0, pph_device_count());
+	napi_value arr;
+	NAPI_OK(napi_create_array_with_length(env, (size_t)n, &arr));
+	for(int d = 0; d < n; d++) {
+		napi_value o = device_object(env, d);
+		if(!o) return nullptr;
+		NAPI_OK(napi_set_element(env, arr, (uint32_t)d, o));
+	}
+	return arr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
 	napi_property_descriptor methods[] = {
 	    {"close", nullptr, Close, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -736,6 +788,11 @@ napi_value Init(napi_env env, napi_value exports) {
 	NAPI_OK(napi_define_class(env, "GfProc", NAPI_AUTO_LENGTH, New, nullptr, sizeof(methods) / sizeof(methods[0]),
 	                          methods, &cls));
 	NAPI_OK(napi_set_named_property(env, exports, "GfProc", cls));
+	napi_value fn;
+	NAPI_OK(napi_create_function(env, "hip_device_info", NAPI_AUTO_LENGTH, HipDeviceInfo, nullptr, &fn));
+	NAPI_OK(napi_set_named_property(env, exports, "hip_device_info", fn));
+	NAPI_OK(napi_create_function(env, "hip_devices", NAPI_AUTO_LENGTH, HipDevices, nullptr, &fn));
+	NAPI_OK(napi_set_named_property(env, exports, "hip_devices", fn));
 	return exports;
 }
 

@@ -9,6 +9,10 @@ const path = require('path');
 
 const binding = require(path.join(__dirname, '..', '..', 'parpar_amd', 'lib', 'gfproc_hip.node'));
 const job = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
+if(job.plan) {  // device ranges from the planner, as lib/par2gen.js gives the OpenCL specs theirs
+	const plan = require(path.join(__dirname, '..', '..', 'parpar_amd', 'node', 'plan.js'));
+	job.hip = plan.planHip({sliceSize: job.slice_size, hipDevices: job.plan}, binding.hip_device_info).devices;
+}
 const input = fs.readFileSync(job.input_file);
 const out = fs.openSync(job.output_file, 'w');
 
@@ -63,7 +67,8 @@ function runPass(i) {
 		const info = gf.info();
 		gf.close(function() {
 			fs.closeSync(out);
-			console.log(JSON.stringify({ok: true, refused: refused, devices: info.hip_devices.length}));
+			console.log(JSON.stringify({ok: true, refused: refused, devices: info.hip_devices.length, hip: job.hip,
+			                             device_info: binding.hip_devices()}));
 		});
 		return;
 	}

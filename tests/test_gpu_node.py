@@ -23,14 +23,14 @@ def say(*a):
     print(*a, flush=True)
 
 
-def run_job(tmp_path, S, rows, passes, hip, staging):
+def run_job(tmp_path, S, rows, passes, hip, staging, plan=None):
     if shutil.which("node") is None or not os.path.exists(BLD.NODE_ADDON):
         pytest.skip("node or the binding is not available")
     inp_file = tmp_path / "inputs.bin"
     out_file = tmp_path / "outputs.bin"
     inp_file.write_bytes(np.stack(rows).tobytes())
     job = {"slice_size": S, "staging": staging, "hip": hip, "input_file": str(inp_file),
-           "output_file": str(out_file), "passes": passes}
+           "output_file": str(out_file), "passes": passes, "plan": plan}
     jf = tmp_path / "job.json"
     jf.write_text(json.dumps(job))
     r = subprocess.run(["node", DRIVER, str(jf)], capture_output=True, text=True, timeout=120)
@@ -103,3 +103,22 @@ def test_node_gfproc_split_and_chunked(tmp_path):
     assert res["devices"] == 2
     for p, (rec, valid) in zip(passes, outs):
         assert (rec == expected(rows, S, p)).all() and valid.all()
+
+
+def test_node_planned_split(tmp_path):
+    """The planner (parpar_amd/node/plan.js) on the visible device: its
+    memory comes from hip_device_info, the whole slice goes to it, and a job
+    run through the planned specs matches the oracle.  (The planner refuses a
+    device listed twice; its arithmetic over several devices is tested on the
+    CPU with fixed device infos.)"""
+    S, n = 5000, 5
+    rows = [O.prng_bytes(53, 0, k, S) for k in range(n)]
+    passes = [{"cur_size": S, "offset": 0, "exps": [0, 3, 11],
+               "inputs": [{"row": k, "num": 2 * k, "len": S} for k in range(n)]}]
+    res, outs = run_job(tmp_path, S, rows, passes, None, staging=2, plan=[{"device": 0, "input_batchsize": 2}])
+    say("planned", res["hip"], res["device_info"])
+    assert res["hip"] == [{"device": 0, "slice_offset": 0, "slice_size": S, "input_batchsize": 2}]
+    assert res["device_info"] and res["device_info"][0]["memory_global"] > (1 << 30)
+    assert res["device_info"][0]["compute_units"] > 0
+    rec, valid = outs[0]
+    assert (rec == expected(rows, S, passes[0])).all() and valid.all()
