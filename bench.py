@@ -327,7 +327,9 @@ def main_backend(args):
     out = np.zeros((R, S), np.uint8)
     be = P.PAR2ProcHIP(0, 2)
     be.setSliceSize(S)
-    be.init(args.cksum, args.batch, parpar_amd.FORM_PLAIN)
+    forms = {"auto": parpar_amd.FORM_AUTO, "plain": parpar_amd.FORM_PLAIN, "bilinear": parpar_amd.FORM_BILINEAR}
+    be.init(args.cksum, args.batch, forms[args.backend_form])
+    in_batch = be.getInputBatchSize()  # the backend's default when --batch is 0
     be.setRecoverySlices(np.arange(R, dtype=np.uint16))
 
     phases = np.zeros(3)  # seconds in the adds, endInput, the getOutputs
@@ -378,7 +380,8 @@ def main_backend(args):
         "data": "synthetic (counter-based random bytes, seed %d), in host memory" % args.seed,
         "config": {"workload": "backend pass: %d host inputs x %d B -> %d recovery slices (exp 0..%d), input batch "
                                "%d, addInput/endInput/getOutput with PCIe both ways" % (
-                                   n, S, R, R - 1, be_batch(args)), "inputs": n, "slice_bytes": S, "recovery": R},
+                                   n, S, R, R - 1, in_batch), "inputs": n, "slice_bytes": S, "recovery": R,
+                   "backend_form": args.backend_form},
         "phases_ms": {"add_inputs": phase_ms[0], "end_input": phase_ms[1], "get_outputs": phase_ms[2]},
         "cpu_check": {"window_bytes": win, "matches_gpu": ok, "mul_adds": n * R * (win // 2),
                       "seconds": round(t_cpu, 2)},
@@ -386,10 +389,6 @@ def main_backend(args):
     print(json.dumps(res), flush=True)
     if not ok:
         raise SystemExit("backend output differs from the oracle")
-
-
-def be_batch(args):
-    return args.batch if args.batch else 64
 
 
 def main_hashes(args):
@@ -591,6 +590,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--form", choices=["plain", "dense", "bilinear", "batched", "serving", "backend", "hashes"], default=None)
     ap.add_argument("--batch", type=int, default=0, help="backend: input batch size (0 = the backend's default)")
+    ap.add_argument("--backend-form", choices=["auto", "plain", "bilinear"], default="auto",
+                    help="backend: kernel form (auto: bilinear for batches of 256 or more inputs with 64 or more "
+                         "recovery slices, else plain)")
     ap.add_argument("--window", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
     ap.add_argument("--bilinear-workspace", type=int, default=40,

@@ -90,10 +90,11 @@ int pph_proc_create(int device, unsigned staging_areas, pph_proc** out);
 /* PAR2ProcOCL::init (gf16/controller_ocl.cpp:67-90): cksum_size is the checksum
  * width of the CPU method the host selects (Galois16Mul::info().cksumSize,
  * gf16/gf16mul.cpp:215-409; 64 on AVX512+GFNI hosts); input_batch 0 picks the
- * default; form is PPH_FORM_PLAIN (plain kernel), _BILINEAR (the bilinear
+ * default (256 inputs, fewer where a staging area would pass 512 MiB); form
+ * is PPH_FORM_PLAIN (plain kernel), _BILINEAR (the bilinear
  * form for batches numbered by input; worth it with large input batches and
  * many recovery slices; explicit-coefficient batches still run the plain
- * kernel) or _AUTO (bilinear for input batches of at least 512 with at least
+ * kernel) or _AUTO (bilinear for input batches of at least 256 with at least
  * 64 recovery slices, plain otherwise; slots prepared by fill_input keep the
  * form chosen then, and a set_recovery_slices that would switch it returns
  * PPH_E_STATE).  setSliceSize must have been called. */

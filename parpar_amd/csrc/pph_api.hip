@@ -539,13 +539,21 @@ struct pph_proc {
 
 namespace {
 
-constexpr unsigned kDefaultBatch = 64;
+// Default input batch: 256 inputs, one bilinear k-block, while a staging
+// area stays within kDefaultAreaBytes (fewer inputs for large slices).  From
+// host buffers, 2000 x 1 MiB with 200 recovery slices took 51.6 ms per pass
+// in 256-input batches (the bilinear form) against 59.4 ms in 64-input ones;
+// with 20 or 64 recovery slices both took 41-44 ms.
+constexpr unsigned kDefaultBatch = 256;
+constexpr size_t kDefaultAreaBytes = 512ull << 20;
 // PPH_FORM_AUTO takes the bilinear form for numbered batches of at least this
-// many inputs with at least this many recovery slices.  2000 inputs x 1 MiB
-// with 200 recovery slices run in 11.7 ms that way against 51.6 ms in the
-// plain kernel; below about 32 recovery slices the plane generation and the
-// row padding cost more than they save.
-constexpr unsigned kAutoBilinearBatch = 512, kAutoBilinearRecovery = 64;
+// many inputs (the bilinear GEMM pads inputs to a multiple of 256) with at
+// least this many recovery slices.  2000 inputs x 1 MiB with 200 recovery
+// slices run in 11.7 ms that way against 51.6 ms in the plain kernel, and a
+// pass from host buffers in 256-input batches in 48 ms against 60 ms in
+// 64-input plain batches; below about 32 recovery slices the plane generation
+// and the row padding cost more than they save.
+constexpr unsigned kAutoBilinearBatch = 256, kAutoBilinearRecovery = 64;
 
 bool bilinear_batches(const pph_proc* p) {
 	if(p->exps.empty()) return false;
@@ -862,6 +870,9 @@ extern "C" int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_ba
 	p->min_batch = p->batch;
 	p->cur_size = p->slice_size;
 	p->slot_bytes = slot_size(p->cur_size, p->ck);
+	if(!input_batch)
+		p->batch = (unsigned)std::max<size_t>(1, std::min<size_t>(kDefaultBatch, kDefaultAreaBytes / p->slot_bytes));
+	p->min_batch = p->batch;
 	p->exps.clear();
 	free_outputs(p);
 	int rc = alloc_areas(p);
