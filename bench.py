@@ -11,7 +11,7 @@ Workloads (--config):
                grouping chosen by the autotuner.
   3            32768 input slices x 64 KiB (seed 3), 8192 recovery slices with
                exponents 0..8191, bilinear dense form (rank-81 Karatsuba formula:
-               81 int8 GEMMs over 0/1 operands; --form dense runs the 16x16
+               81 GEMMs over 0/1 operands on fp4 MFMA; --form dense runs the 16x16
                bit-matrix MFMA kernel instead), with the plain form timed on the
                same data for the ratio.
   4            serving: files arrive as a Poisson process (20/s, seed 4) for a
@@ -798,18 +798,15 @@ def main():
 
     if dense:
         # MFMA ops per mul-add: 2 x 256 MACs (16x16 bit-matrix, int8) or 2 x 81 (bilinear
-        # formula: fp4 e2m1 MFMA kernel, or int8 GEMMs with PPH_BILINEAR_GEMM=library)
+        # formula on the fp4 e2m1 MFMA kernel)
         per_ma = 512 if args.form == "dense" else 162
-        library = os.environ.get("PPH_BILINEAR_GEMM") == "library"
-        fp4 = args.form == "bilinear" and not library
+        fp4 = args.form == "bilinear"
         peak = FP4_PEAK_TFLOPS if fp4 else INT8_PEAK_TOPS
         achieved = per_ma * ma / (ker_avg / 1e3) / 1e12
         if args.form == "dense":
             kname = "k_muladd_dense (int8 MFMA)"
-        elif fp4:
-            kname = "pph_muladd_bilinear: k_blm_gemm (fp4 MFMA) + plane and recombination kernels"
         else:
-            kname = "pph_muladd_bilinear: 81 batched int8 GEMMs (hipBLASLt) + plane and parity kernels"
+            kname = "pph_muladd_bilinear: k_blm_gemm (fp4 MFMA) + plane and recombination kernels"
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s" if fp4 else "TOP/s",
                 "frac": round(achieved / peak, 4), "traffic": MEASURED_TRAFFIC.get((args.form, n, S, R)),
                 "kernel": kname, "kernel_ms": round(ker_avg, 3), "algorithmic_mfma_ops": per_ma * ma,

@@ -58,7 +58,7 @@ extern "C" {
 #define PPH_FORM_AUTO 0
 #define PPH_FORM_PLAIN 1   /* LDS log/exp streaming kernel */
 #define PPH_FORM_DENSE 2   /* GF(2) bit-matrix contraction on int8 MFMA */
-#define PPH_FORM_BILINEAR 3  /* rank-81 bilinear formula as batched int8 GEMMs (pph_muladd_bilinear) */
+#define PPH_FORM_BILINEAR 3  /* rank-81 bilinear formula on fp4 MFMA over 0/1 bit planes (pph_muladd_bilinear) */
 
 const char* pph_strerror(int status);
 int pph_last_hip_error(void);
@@ -117,8 +117,9 @@ int pph_proc_flush(pph_proc* p);                                      /* flush (
 /* endInput (:179): flushes and returns when every submitted batch has completed */
 int pph_proc_end_input(pph_proc* p);
 /* getOutput (:185): copies current_slice_size bytes of recovery slice `index`
- * into output; *valid = gf16_cksum check of the slice (copy_cksum_check,
- * gf16/gf16_cksum_base.h:148-159). */
+ * into output; *valid = gf16_cksum check (copy_cksum_check,
+ * gf16/gf16_cksum_base.h:48-59) computed on the host over the bytes as they
+ * are copied into output, against the checksum block the mul-add produced. */
 int pph_proc_get_output(pph_proc* p, unsigned index, void* output, int* valid);
 void pph_proc_discard_output(pph_proc* p);                            /* discardOutput (:186) */
 int pph_proc_has_added(const pph_proc* p);                            /* _hasAdded (:189) */
@@ -131,6 +132,52 @@ int pph_proc_num_recovery_slices(const pph_proc* p);                  /* getNumR
 /* Fault injection for tests: XOR `mask` into byte `offset` of recovery slice
  * `index` on the device (the checksum must then report the slice invalid). */
 int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t offset, uint8_t mask);
+/* Fault injection for tests: XOR `mask` into byte `offset` of the next
+ * recovery slice getOutput copies, in host memory after its device-to-host
+ * copy and before the checked copy to the caller (that slice must then be
+ * reported invalid while the device copy stays intact). */
+int pph_proc_debug_corrupt_host_copy(pph_proc* p, size_t offset, uint8_t mask);
+/* Fault injection for tests: the next batch submission fails with `status`
+ * (negative).  A failed submission loses its inputs; the backend then keeps
+ * that status and returns it from endInput and every getOutput (valid = 0)
+ * until a new recovery computation starts (discardOutput, init, a grown
+ * slice, freeProcessingMem). */
+int pph_proc_debug_fail_next_batch(pph_proc* p, int status);
+
+/* ---- Asynchronous form (the libuv build of the interface, controller.h:79-100,
+ * 147-152, 169-185): the caller keeps the staging state on its own thread, as
+ * PAR2ProcOCL does (controller_ocl.cpp:303-350,405-426), and names the area
+ * and slot of every input; completions come back through these functions,
+ * called from the backend's worker thread (they should only post to the
+ * caller's loop, as kernel_event_callback does, controller_ocl.cpp:485-494).
+ * Staging and submission calls come from one thread, and none overlaps a
+ * pending getOutput. */
+typedef void (*pph_done_fn)(void* user, int status);
+typedef void (*pph_output_fn)(void* user, int status, int valid);
+unsigned pph_proc_staging_areas(const pph_proc* p);
+/* Copy one input (size <= current slice size, zero padded) into slot `slot`
+ * of staging area `area` with its PAR2 input number, or with coeffs[r] for
+ * every recovery slice r when coeffs is not NULL (the two addInput forms,
+ * controller.h:169-170).  Returns once the caller's buffer may be reused;
+ * waits first if the area's previous batch is still running. */
+int pph_proc_stage_input(pph_proc* p, unsigned area, unsigned slot, const void* buffer, size_t size,
+                         uint16_t input_num, const uint16_t* coeffs);
+/* dummyInput (:171): the slot keeps its prepared bytes (pph_proc_fill_slot),
+ * only its input number is set. */
+int pph_proc_stage_dummy(pph_proc* p, unsigned area, unsigned slot, uint16_t input_num);
+/* fillInput (:172) into an explicit slot. */
+int pph_proc_fill_slot(pph_proc* p, unsigned area, unsigned slot, const void* buffer);
+/* run_kernel (:85): the area's first n_inputs slots as one batch.  first != 0
+ * starts a new recovery computation (overwrite; the processingAdd = false of
+ * discardOutput, :186-188); otherwise the batch accumulates.  done(user,
+ * status) runs exactly once: when the batch's mul-add has completed, or when
+ * the submission failed (its status; the backend then keeps that error). */
+int pph_proc_submit(pph_proc* p, unsigned area, unsigned n_inputs, int first, pph_done_fn done, void* user);
+/* getOutput (:185) run on the worker thread after every task posted before
+ * it: done(user, status, valid) once `output` holds the slice. */
+int pph_proc_get_output_async(pph_proc* p, unsigned index, void* output, pph_output_fn done, void* user);
+/* Wait until every posted completion function has returned. */
+int pph_proc_drain(pph_proc* p);
 
 /* ======================================================================
  * Device layer
@@ -146,6 +193,11 @@ int pph_memset(void* dst, int value, size_t bytes);
 /* Device-to-device copy by a plain 16-byte streaming kernel (bytes and both
  * pointers 16-byte aligned); the reference point for HBM traffic counters. */
 int pph_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* Ceiling of the plain form's inner operation, measured: random 16-bit
+ * lookups into the 128 KiB exp table in LDS, one 1024-thread workgroup per CU
+ * (as the plain kernel runs), 16 lookups per lane per round for `rounds`
+ * rounds.  Lookups per second of the timed launch (and its time in ms). */
+int pph_lds_gather_ceiling(unsigned rounds, double* lookups_per_s, float* ms);
 int pph_synchronize(void);
 /* Streams are opaque HIP streams at default priority; NULL = the null stream. */
 int pph_stream_create(void** stream);
@@ -163,7 +215,7 @@ int pph_timer_elapsed_ms(void* timer, float* ms);
 int pph_fill_random(void* slots, size_t slot_bytes, unsigned count, size_t len,
                     uint64_t seed, uint32_t job, uint32_t first_slice, void* stream);
 
-/* Input preparation, the device half of copy_cksum (gf16/gf16_cksum_base.h:132-146):
+/* Input preparation, the device half of copy_cksum (gf16/gf16_cksum_base.h:32-46):
  * for slot k, bytes at or past lens[k] (device uint32 array; NULL = slice_size)
  * read as zero, the gf16_cksum of the slice_size-byte slice is stored at ck_off,
  * and, with to_log, every word of the slot is converted to log domain. */
@@ -194,9 +246,11 @@ int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, con
 
 /* Bilinear dense form: the same result as pph_muladd_plain / pph_muladd_dense
  * (data-form input slots, PAR2 coefficients), computed from a rank-81 bilinear
- * formula for GF(2^16) multiplication (four levels of Karatsuba): 81 batched
- * int8 GEMMs over 0/1 operands (hipBLASLt) between hand-written expansion and
- * parity/recombination kernels; 81 MACs per mul-add instead of 256.  Needs a
+ * formula for GF(2^16) multiplication (four levels of Karatsuba): 81 integer
+ * GEMMs over 0/1 operands, run as one persistent fp4 MFMA kernel over
+ * bit-packed planes (exact: the fp32 sums are integers below 2^24), between
+ * plane-building and parity/recombination kernels; 81 MACs per mul-add
+ * instead of 256.  Needs a
  * device workspace (pph_bilinear_workspace bytes), allocated on first use,
  * kept for later calls and freed by pph_bilinear_release.  Calls on one device
  * share it: each call's device work starts after the previous call's ends,
@@ -205,6 +259,10 @@ int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, 
                         const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes);
 int pph_bilinear_release(void);
+/* Test hook: allocations of the shared bilinear workspace above `bytes` fail
+ * with PPH_E_NOMEM (0 = no limit).  The backend then runs such batches in the
+ * plain kernel (the same bytes). */
+int pph_debug_bilinear_workspace_limit(size_t bytes);
 /* Host reference of the bilinear formula: c * x in GF(2^16) through the
  * expansion, componentwise product, recombination and reduction. */
 uint16_t pph_bilinear_mul(uint16_t c, uint16_t x);
@@ -227,7 +285,7 @@ int pph_slice_hash_ifsc(const void* slots, size_t slot_bytes, unsigned count, co
 int pph_file_md5(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size, uint64_t file_size,
                  uint8_t* md5_out, void* stream);
 
-/* gf16_cksum verification of n slots (copy_cksum_check, gf16/gf16_cksum_base.h:148-159):
+/* gf16_cksum verification of n slots (copy_cksum_check, gf16/gf16_cksum_base.h:48-59):
  * valid[k] (device int array) = 1 if the stored block matches the slice data. */
 int pph_verify_slots(const void* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                      unsigned cksum_size, int* valid, void* stream);
@@ -278,9 +336,7 @@ int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group,
 /* Run the batch on a stream: counter reset, per-job preparation, the fused
  * persistent mul-add (and dense / bilinear launches), per-job verification.
  * use_graph captures that sequence into a hipGraph on first use and replays
- * it; the bilinear jobs' library GEMMs and their checks are enqueued after the
- * replay, outside the graph.  A batch with bilinear jobs owns a device
- * workspace for them (the largest job's pph_bilinear_workspace). */
+ * it.  A batch with bilinear jobs owns a device workspace for them. */
 int pph_batch_run(pph_batch* b, int use_graph, void* stream);
 /* Time of the mul-add launches of the last eager (use_graph = 0) run, from HIP
  * events recorded on its stream around them; waits for that run. */

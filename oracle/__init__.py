@@ -14,6 +14,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(_HERE, "liboracle.so")
 REF_SO = os.path.join(_HERE, "_ref", "libparpar_ref.so")
+# the same harness over a portable build of the reference (per-file ISA flags, run-time
+# method choice by CPUID): the one to run on a host other than the build container
+REF_PORTABLE_SO = os.path.join(_HERE, "_ref", "libparpar_ref_portable.so")
 
 _c_u8p = ctypes.POINTER(ctypes.c_uint8)
 _c_u16p = ctypes.POINTER(ctypes.c_uint16)
@@ -72,17 +75,22 @@ def lib():
     return _lib
 
 
-def ref_available():
-    return os.path.exists(REF_SO)
+def ref_available(portable=False):
+    return os.path.exists(REF_PORTABLE_SO if portable else REF_SO)
 
 
-def ref():
-    """Load the reference harness (only present where /root/reference was built)."""
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise RuntimeError("oracle/_ref/libparpar_ref.so is not built (run `make -C oracle ref`)")
-        R = ctypes.CDLL(REF_SO)
+_ref_portable = None
+
+
+def ref(portable=False):
+    """Load the reference harness (only present where /root/reference was built).
+    portable=True loads the build that runs on any x86-64 host."""
+    global _ref, _ref_portable
+    if (_ref_portable if portable else _ref) is None:
+        path = REF_PORTABLE_SO if portable else REF_SO
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (run `make -C oracle ref portable`)" % path)
+        R = ctypes.CDLL(path)
         R.ref_info.restype = ctypes.c_int
         R.ref_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint),
                                ctypes.POINTER(ctypes.c_uint)]
@@ -102,8 +110,11 @@ def ref():
         R.ref_recover.argtypes = [ctypes.c_int, ctypes.c_uint, _c_u8p, ctypes.c_size_t, _c_szp, _c_u16p, _c_u16p,
                                   ctypes.c_uint, _c_u16p, ctypes.c_size_t, ctypes.c_size_t, _c_u8p, _c_intp,
                                   ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
-        _ref = R
-    return _ref
+        if portable:
+            _ref_portable = R
+        else:
+            _ref = R
+    return _ref_portable if portable else _ref
 
 
 # ---------------------------------------------------------------- helpers
@@ -155,9 +166,10 @@ def cksum_verify(data, ck, width):
     return bool(L.po_cksum_verify(_ptr(data), data.shape[0], _ptr(ck), width))
 
 
-def ref_recover(inputs, in_lens, slice_size, exps, input_nums=None, coeffs=None, split=0, threads=0, method=0):
+def ref_recover(inputs, in_lens, slice_size, exps, input_nums=None, coeffs=None, split=0, threads=0, method=0,
+                portable=False):
     """Run the reference PAR2Proc + PAR2ProcCPU; returns (outputs, valid, seconds)."""
-    R = ref()
+    R = ref(portable)
     inputs = np.ascontiguousarray(inputs, dtype=np.uint8)
     n_in = inputs.shape[0]
     lens = np.ascontiguousarray(in_lens, dtype=np.uintp)
@@ -178,8 +190,8 @@ def ref_recover(inputs, in_lens, slice_size, exps, input_nums=None, coeffs=None,
     return out, valid.astype(bool), secs.value
 
 
-def ref_info():
-    R = ref()
+def ref_info(portable=False):
+    R = ref(portable)
     name = ctypes.create_string_buffer(64)
     ck = ctypes.c_uint(0)
     th = ctypes.c_uint(0)

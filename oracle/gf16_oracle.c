@@ -7,12 +7,12 @@
  *   po_mul / po_exp / po_log    gf16/gf16_global.h:9-10 (polynomial 0x1100B, generator 2)
  *   po_input_log                gf16/gfmat_coeff.c:12-22,56-58 (PAR2 sieve)
  *   po_coeff_log / po_coeff     gf16/gfmat_coeff.c:60-77
- *   po_cksum / po_cksum_verify  gf16/gf16_cksum_base.h:113-159 with the lane-wise
+ *   po_cksum / po_cksum_verify  gf16/gf16_cksum_base.h:13-59 with the lane-wise
  *                               Horner step of gf16/gf16_checksum_generic.h:25-124
  *                               and gf16/gf16_checksum_x86.h:6-150
  *   po_muladd_region            gf16/gf16_lookup.c:124-249 (split 8-bit tables)
  *   po_recover                  gf16/controller_cpu.cpp:463-537 (+ zero-pad of short
- *                               inputs, gf16/gf16_cksum_base.h:138-141)
+ *                               inputs, gf16/gf16_cksum_base.h:38-44)
  *   po_md5 / po_crc32           hasher/hasher_base.h:83-198 semantics, RFC 1321 and
  *                               IEEE CRC-32 (reflected 0xEDB88320)
  *

@@ -30,7 +30,7 @@ uint16_t po_input_log(uint16_t input);
 uint16_t po_coeff_log(uint16_t inputLog, uint16_t exp);
 uint16_t po_coeff(uint16_t input, uint16_t exp);
 
-/* gf16_cksum (gf16/gf16_cksum_base.h:113-159, gf16/gf16_checksum_generic.h) */
+/* gf16_cksum (gf16/gf16_cksum_base.h:13-59, gf16/gf16_checksum_generic.h) */
 void po_cksum(const uint8_t* src, size_t srcLen, size_t sliceLen, unsigned width, uint8_t* ck);
 int po_cksum_verify(const uint8_t* data, size_t len, const uint8_t* ck, unsigned width);
 

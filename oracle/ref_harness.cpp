@@ -49,7 +49,7 @@ uint16_t ref_gf16_exp(uint16_t v) {
 }
 
 /* Galois16Mul::copy_cksum / copy_cksum_check of a given method (0 = AUTO),
- * i.e. gf16_cksum_copy / gf16_cksum_copy_check (gf16/gf16_cksum_base.h:132-159).
+ * i.e. gf16_cksum_copy / gf16_cksum_copy_check (gf16/gf16_cksum_base.h:13-59).
  * dst must hold sliceLen + cksumSize bytes.  Returns the checksum width used. */
 int ref_cksum_copy(int method, void* dst, const void* src, size_t srcLen, size_t sliceLen) {
 	gfmat_init();

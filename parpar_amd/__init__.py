@@ -74,6 +74,16 @@ _SIGS = {
     "pph_proc_set_min_input_batch_size": (None, [_vp, _u]),
     "pph_proc_num_recovery_slices": (_i, [_vp]),
     "pph_proc_debug_corrupt_output": (_i, [_vp, _u, _sz, ctypes.c_uint8]),
+    "pph_proc_debug_corrupt_host_copy": (_i, [_vp, _sz, ctypes.c_uint8]),
+    "pph_proc_debug_fail_next_batch": (_i, [_vp, _i]),
+    "pph_debug_bilinear_workspace_limit": (_i, [_sz]),
+    "pph_proc_staging_areas": (_u, [_vp]),
+    "pph_proc_stage_input": (_i, [_vp, _u, _u, _vp, _sz, _u16, _vp]),
+    "pph_proc_stage_dummy": (_i, [_vp, _u, _u, _u16]),
+    "pph_proc_fill_slot": (_i, [_vp, _u, _u, _vp]),
+    "pph_proc_submit": (_i, [_vp, _u, _u, _i, _vp, _vp]),
+    "pph_proc_get_output_async": (_i, [_vp, _u, _vp, _vp, _vp]),
+    "pph_proc_drain": (_i, [_vp]),
     "pph_set_device": (_i, [_i]),
     "pph_mem_get_info": (_i, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
     "pph_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
@@ -82,6 +92,7 @@ _SIGS = {
     "pph_memcpy_d2h": (_i, [_vp, _vp, _sz]),
     "pph_memset": (_i, [_vp, _i, _sz]),
     "pph_copy": (_i, [_vp, _vp, _sz, _vp]),
+    "pph_lds_gather_ceiling": (_i, [_u, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
     "pph_synchronize": (_i, []),
     "pph_stream_create": (_i, [ctypes.POINTER(_vp)]),
     "pph_stream_destroy": (_i, [_vp]),

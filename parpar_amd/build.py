@@ -21,7 +21,6 @@ INCLUDE = os.path.join(ROOT, "include")
 
 ARCH = os.environ.get("PPH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-ROCM_LIB = "/opt/rocm/lib"
 HIP_SOURCES = ["pph_kernels.hip", "pph_dense.hip", "pph_batch.hip", "pph_hash.hip", "pph_serve.hip", "pph_bilinear.hip", "pph_bl_mfma.hip",
                "pph_api.hip"]
 CXX_SOURCES = ["controller_hip.cpp"]
@@ -43,19 +42,22 @@ def _run(cmd, cwd=None):
 def build_library(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC, "-c",
-              os.path.join(CSRC, src), "-o", obj])
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC, "-c",
+                     os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # translation units compile independently: a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-         ["-L" + ROCM_LIB, "-lhipblaslt", "-Wl,-rpath," + ROCM_LIB])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
@@ -69,6 +71,7 @@ def build_oracle(verbose=False):
     if os.path.isdir("/root/reference/gf16"):
         _run(["make", "-C", odir, "-j8", "ref"])
         _run(["make", "-C", odir, "integration"])
+        _run(["make", "-C", odir, "-j8", "uv", "portable"])
     if verbose:
         print("built oracle")
 

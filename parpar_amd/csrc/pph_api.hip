@@ -6,7 +6,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
+#include <thread>
 #include <cstring>
 #include <cstdlib>
 #include <memory>
@@ -237,6 +240,33 @@ extern "C" int pph_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 extern "C" int pph_copy(void* dst, const void* src, size_t bytes, void* stream) {
 	if(!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return PPH_E_INVALID;
 	PPH_HIP(launch_copy(dst, src, bytes, (hipStream_t)stream));
+	return PPH_OK;
+}
+extern "C" int pph_lds_gather_ceiling(unsigned rounds, double* lookups_per_s, float* ms) {
+	if(!rounds || !lookups_per_s) return PPH_E_INVALID;
+	DeviceTables* t;
+	int rc = device_tables(&t);
+	if(rc) return rc;
+	const unsigned blocks = (unsigned)(t->cus > 0 ? t->cus : 256);
+	uint32_t* sink = nullptr;
+	PPH_HIP(hipMalloc((void**)&sink, (size_t)blocks * kPlainThreads * sizeof(uint32_t)));
+	hipEvent_t a = nullptr, b = nullptr;
+	hipError_t e = hipEventCreate(&a);
+	if(e == hipSuccess) e = hipEventCreate(&b);
+	// a first launch warms the code object and the table; the second is timed
+	if(e == hipSuccess) e = launch_lds_gather_ceiling(t->exp, blocks, rounds, 12345u, sink, nullptr);
+	if(e == hipSuccess) e = hipEventRecord(a, nullptr);
+	if(e == hipSuccess) e = launch_lds_gather_ceiling(t->exp, blocks, rounds, 777u, sink, nullptr);
+	if(e == hipSuccess) e = hipEventRecord(b, nullptr);
+	if(e == hipSuccess) e = hipEventSynchronize(b);
+	float t_ms = 0;
+	if(e == hipSuccess) e = hipEventElapsedTime(&t_ms, a, b);
+	if(a) (void)hipEventDestroy(a);
+	if(b) (void)hipEventDestroy(b);
+	(void)hipFree(sink);
+	if(e != hipSuccess) return set_error(e);
+	*lookups_per_s = (double)blocks * kPlainThreads * 16.0 * rounds / (t_ms * 1e-3);
+	if(ms) *ms = t_ms;
 	return PPH_OK;
 }
 extern "C" int pph_memset(void* dst, int value, size_t bytes) {
@@ -523,18 +553,40 @@ struct pph_proc {
 	unsigned r_pad = 0, r_alloc = 0;
 	uint8_t* d_out = nullptr;
 	uint16_t* d_exps = nullptr;
-	int* d_valid = nullptr;
-	std::vector<int> h_valid;
-	bool verified = false;
+	int failed = PPH_OK;   // first error of a batch submission: its inputs are lost, so the
+	                       // recovery slices are incomplete until the next one starts afresh
 	unsigned cur_area = 0, cur_inputs = 0;
 	unsigned active_count = 0;
 	unsigned batches_started = 0;
 	bool processing_add = false;
 	std::unique_ptr<pph::CopyPool> copier;  // host copies into and out of pinned memory
-	uint8_t* h_bounce[2] = {nullptr, nullptr};  // pinned, one slice each: getOutput's device-to-host copies
+	uint8_t* h_bounce[2] = {nullptr, nullptr};  // pinned, one slot each: getOutput's device-to-host copies
 	size_t bounce_bytes = 0;
+	std::vector<uint8_t> h_pageable;  // getOutput's bounce where pinned memory is not available
 	int pf_index = -1;     // recovery slice already copied (or in flight on s_comp) into h_bounce[pf_buf]
 	unsigned pf_buf = 0;
+	size_t dbg_flip_off = 0;  // test hook: one byte of the next bounced slice flipped on the host
+	uint8_t dbg_flip_mask = 0;
+	int dbg_fail = PPH_OK;    // test hook: the next batch submission fails with this status
+
+	// Asynchronous entry points: a worker thread waits for submitted batches
+	// and runs queued getOutputs, in the order they were posted, and calls the
+	// caller's completion functions from there.
+	struct Task {
+		hipEvent_t ev = nullptr;        // batch: recorded after its mul-add (null: failed submission)
+		int status = PPH_OK;
+		pph_done_fn done = nullptr;
+		pph_output_fn out_done = nullptr;  // set for a getOutput task
+		void* user = nullptr;
+		unsigned index = 0;
+		void* output = nullptr;
+	};
+	std::mutex wmu;
+	std::condition_variable wcv;
+	std::deque<Task> tasks;
+	std::vector<hipEvent_t> free_events;
+	bool wstop = false, wbusy = false;
+	std::thread worker;
 };
 
 namespace {
@@ -584,10 +636,8 @@ void free_areas(pph_proc* p) {
 void free_outputs(pph_proc* p) {
 	if(p->d_out) (void)hipFree(p->d_out);
 	if(p->d_exps) (void)hipFree(p->d_exps);
-	if(p->d_valid) (void)hipFree(p->d_valid);
 	p->d_out = nullptr;
 	p->d_exps = nullptr;
-	p->d_valid = nullptr;
 	p->r_alloc = 0;
 	p->pf_index = -1;
 }
@@ -659,16 +709,29 @@ int flush_h2d(pph_proc* p, Area& a) {
 	return PPH_OK;
 }
 
-int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
+int submit_area(pph_proc* p, unsigned area_idx, unsigned n) {
 	Area& a = p->areas[area_idx];
+	if(p->dbg_fail) {
+		const int rc = p->dbg_fail;
+		p->dbg_fail = PPH_OK;
+		return rc;
+	}
 	if(int rc = flush_h2d(p, a)) return rc;
 	uint32_t *lens, *slots;
 	uint16_t* logs;
 	meta_ptrs(p, a, &lens, &slots, &logs);
 	const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
 	// the bilinear form takes data-form inputs; batches with explicit
-	// coefficients always run the plain form
-	const bool bl = bilinear_batches(p) && !a.explicit_coef;
+	// coefficients always run the plain form, and so does a batch whose
+	// bilinear workspace cannot be allocated (the same bytes, more time)
+	bool bl = bilinear_batches(p) && !a.explicit_coef;
+	if(bl && a.filled_form != 1) {
+		const int rc = bilinear_reserve(p->slot_bytes, n, (unsigned)p->exps.size());
+		if(rc == PPH_E_NOMEM)
+			bl = false;
+		else if(rc)
+			return rc;
+	}
 	if(a.filled_form >= 0 && !p->exps.empty() && a.filled_form != (bl ? 1 : 0))
 		return PPH_E_STATE;  // fillInput prepared this area's slots for the other form
 	PPH_HIP(hipMemcpyAsync(a.d_meta, a.h_meta, meta, hipMemcpyHostToDevice, p->s_copy));
@@ -702,9 +765,29 @@ int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
 	a.explicit_coef = false;
 	p->active_count++;
 	p->batches_started++;
-	p->verified = false;
 	p->pf_index = -1;
 	return PPH_OK;
+}
+
+// Submit a batch; a failed submission loses its inputs, so the backend keeps
+// the first such error and reports it from endInput and every getOutput until
+// a new recovery computation starts (discardOutput, init, a grown slice).
+int run_kernel(pph_proc* p, unsigned area_idx, unsigned n) {
+	const int rc = submit_area(p, area_idx, n);
+	if(rc == PPH_OK) return PPH_OK;
+	Area& a = p->areas[area_idx];
+	// whatever was enqueued for the area finishes before it is refilled
+	(void)hipStreamSynchronize(p->s_copy);
+	(void)hipStreamSynchronize(p->s_comp);
+	(void)hipGetLastError();
+	a.active = false;
+	a.explicit_coef = false;
+	a.prep_count = 0;
+	a.h2d_count = 0;
+	p->batches_started++;
+	p->pf_index = -1;
+	if(p->failed == PPH_OK) p->failed = rc;
+	return rc;
 }
 
 void advance_area(pph_proc* p) {
@@ -745,15 +828,11 @@ void to_explicit(pph_proc* p, Area& a, unsigned upto) {
 			    r < p->exps.size() ? 2u * host_coeff_log(logs[k], p->exps[r]) : kSkipCoef;
 }
 
-int add_common(pph_proc* p, const void* buffer, size_t size, const uint16_t* coeffs, uint16_t input_num, int flush,
-               bool dummy) {
-	if(!p || !p->inited) return PPH_E_STATE;
-	if(!dummy && size && !buffer) return PPH_E_INVALID;
-	if(size > p->cur_size) return PPH_E_INVALID;
-	Area& a = p->areas[p->cur_area];
-	int rc = wait_area(p, a);
-	if(rc) return rc;
-	const unsigned idx = p->cur_inputs;
+// Stage one input into slot idx of an area that is not active: its
+// coefficient data, and (unless dummy) its bytes into the pinned staging,
+// queued for preparation and sent to the device in contiguous runs.
+int stage_slot(pph_proc* p, Area& a, unsigned idx, const void* buffer, size_t size, const uint16_t* coeffs,
+               uint16_t input_num, bool dummy) {
 	uint32_t *lens, *slots;
 	uint16_t* logs;
 	meta_ptrs(p, a, &lens, &slots, &logs);
@@ -773,18 +852,31 @@ int add_common(pph_proc* p, const void* buffer, size_t size, const uint16_t* coe
 		uint8_t* stage = a.h_stage + (size_t)idx * p->slot_bytes;
 		if(size) p->copier->copy(stage, buffer, size);
 		if(a.h2d_count && a.h2d_first + a.h2d_count != idx) {
-			rc = flush_h2d(p, a);
+			const int rc = flush_h2d(p, a);
 			if(rc) return rc;
 		}
 		if(!a.h2d_count) a.h2d_first = idx;
 		if(++a.h2d_count == kH2DRun) {
-			rc = flush_h2d(p, a);
+			const int rc = flush_h2d(p, a);
 			if(rc) return rc;
 		}
 		lens[a.prep_count] = (uint32_t)size;
 		slots[a.prep_count] = idx;
 		a.prep_count++;
 	}
+	return PPH_OK;
+}
+
+int add_common(pph_proc* p, const void* buffer, size_t size, const uint16_t* coeffs, uint16_t input_num, int flush,
+               bool dummy) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(!dummy && size && !buffer) return PPH_E_INVALID;
+	if(size > p->cur_size) return PPH_E_INVALID;
+	Area& a = p->areas[p->cur_area];
+	int rc = wait_area(p, a);
+	if(rc) return rc;
+	rc = stage_slot(p, a, p->cur_inputs, buffer, size, coeffs, input_num, dummy);
+	if(rc) return rc;
 	p->cur_inputs++;
 	poll_areas(p);
 	if(should_submit(p, flush != 0)) {
@@ -838,7 +930,17 @@ extern "C" int pph_proc_create(int device, unsigned staging_areas, pph_proc** ou
 
 extern "C" void pph_proc_destroy(pph_proc* p) {
 	if(!p) return;
+	if(p->worker.joinable()) {
+		// the worker runs every queued task (its completion functions included) first
+		{
+			std::lock_guard<std::mutex> lk(p->wmu);
+			p->wstop = true;
+		}
+		p->wcv.notify_all();
+		p->worker.join();
+	}
 	(void)hipSetDevice(p->device);
+	for(hipEvent_t e : p->free_events) (void)hipEventDestroy(e);
 	if(p->s_comp) (void)hipStreamSynchronize(p->s_comp);
 	if(p->s_copy) (void)hipStreamSynchronize(p->s_copy);
 	free_areas(p);
@@ -881,7 +983,7 @@ extern "C" int pph_proc_init(pph_proc* p, unsigned cksum_size, unsigned input_ba
 	p->active_count = 0;
 	p->batches_started = 0;
 	p->processing_add = false;
-	p->verified = false;
+	p->failed = PPH_OK;
 	p->pf_index = -1;
 	p->r_pad = 0;
 	p->inited = true;
@@ -903,7 +1005,7 @@ extern "C" int pph_proc_set_current_slice_size(pph_proc* p, size_t slice_size) {
 		// starts the recovery afresh (processingAdd = false, :147)
 		p->slice_size = std::max(p->slice_size, slice_size);
 		p->processing_add = false;
-		p->verified = false;
+		p->failed = PPH_OK;
 		p->pf_index = -1;
 		p->cur_area = p->cur_inputs = 0;
 		p->active_count = 0;
@@ -949,7 +1051,6 @@ extern "C" int pph_proc_set_recovery_slices(pph_proc* p, unsigned n, const uint1
 		free_outputs(p);
 		if(hipMalloc(&p->d_out, (size_t)r_pad * p->alloc_slot_bytes) != hipSuccess) return PPH_E_NOMEM;
 		if(hipMalloc(&p->d_exps, (size_t)r_pad * sizeof(uint16_t)) != hipSuccess) return PPH_E_NOMEM;
-		if(hipMalloc(&p->d_valid, (size_t)r_pad * sizeof(int)) != hipSuccess) return PPH_E_NOMEM;
 		p->r_alloc = r_pad;
 	}
 	if(r_pad != p->r_pad) {
@@ -960,8 +1061,6 @@ extern "C" int pph_proc_set_recovery_slices(pph_proc* p, unsigned n, const uint1
 	std::vector<uint16_t> ex(r_pad, 0);
 	std::copy(p->exps.begin(), p->exps.end(), ex.begin());
 	PPH_HIP(hipMemcpy(p->d_exps, ex.data(), r_pad * sizeof(uint16_t), hipMemcpyHostToDevice));
-	p->h_valid.assign(r_pad, 0);
-	p->verified = false;
 	p->pf_index = -1;
 	return PPH_OK;
 }
@@ -1034,64 +1133,75 @@ extern "C" int pph_proc_end_input(pph_proc* p) {
 	PPH_HIP(hipStreamSynchronize(p->s_copy));
 	PPH_HIP(hipStreamSynchronize(p->s_comp));
 	poll_areas(p);
-	return PPH_OK;
+	return p->failed;
 }
 
+namespace {
+
+// Pinned bounce slots for getOutput (the slice and its checksum block).
+void ensure_bounce(pph_proc* p) {
+	if(p->bounce_bytes >= p->alloc_slot_bytes) return;
+	for(uint8_t*& b : p->h_bounce) {
+		if(b) (void)hipHostFree(b);
+		b = nullptr;
+	}
+	p->bounce_bytes = 0;
+	p->pf_index = -1;
+	bool ok = true;
+	for(uint8_t*& b : p->h_bounce)
+		ok = ok && hipHostMalloc((void**)&b, p->alloc_slot_bytes, hipHostMallocDefault) == hipSuccess;
+	if(ok) p->bounce_bytes = p->alloc_slot_bytes;
+	(void)hipGetLastError();
+}
+
+}  // namespace
+
 extern "C" int pph_proc_get_output(pph_proc* p, unsigned index, void* output, int* valid) {
+	if(valid) *valid = 0;
 	if(!p || !p->inited) return PPH_E_STATE;
 	if(index >= p->exps.size() || (!output && p->cur_size)) return PPH_E_INVALID;
+	if(p->failed) return p->failed;
 	if(!p->processing_add) {
 		// nothing computed on this backend: zero fill (controller.cpp:376-382)
 		if(p->cur_size) memset(output, 0, p->cur_size);
 		if(valid) *valid = 1;
 		return PPH_OK;
 	}
+	// The slice and its checksum block come to host memory, and the checksum is
+	// recomputed over the bytes as they are copied to the caller (the check of
+	// copy_cksum_check, gf16/gf16_cksum_base.h:48-59, on the delivered bytes).
+	const size_t ck_off = ck_offset(p->cur_size), span = ck_off + p->ck;
 	PPH_HIP(hipStreamSynchronize(p->s_comp));
-	if(!p->verified) {
-		PPH_HIP(launch_verify(p->d_out, p->slot_bytes, (unsigned)p->exps.size(), p->cur_size, ck_offset(p->cur_size),
-		                      p->ck, p->d_valid, *p->tab, p->s_comp));
-		PPH_HIP(hipMemcpyAsync(p->h_valid.data(), p->d_valid, p->exps.size() * sizeof(int), hipMemcpyDeviceToHost,
-		                       p->s_comp));
-		PPH_HIP(hipStreamSynchronize(p->s_comp));
-		p->verified = true;
-	}
-	if(p->cur_size) {
-		// through pinned bounce slices (full-rate DMA) and a parallel host copy;
-		// the next slice's device-to-host copy runs during this slice's host copy
-		if(p->bounce_bytes < p->cur_size) {
-			for(uint8_t*& b : p->h_bounce) {
-				if(b) (void)hipHostFree(b);
-				b = nullptr;
-			}
-			p->bounce_bytes = 0;
-			p->pf_index = -1;
-			bool ok = true;
-			for(uint8_t*& b : p->h_bounce)
-				ok = ok && hipHostMalloc((void**)&b, p->alloc_slice, hipHostMallocDefault) == hipSuccess;
-			if(ok) p->bounce_bytes = p->alloc_slice;
-			(void)hipGetLastError();
+	ensure_bounce(p);
+	const uint8_t* src = p->d_out + (size_t)index * p->slot_bytes;
+	const uint8_t* host;
+	if(p->bounce_bytes >= span) {
+		// pinned bounce slots (full-rate DMA); the next slice's device-to-host
+		// copy runs during this slice's host copy
+		const unsigned b = p->pf_buf;
+		if(p->pf_index != (int)index) {  // (a prefetched slice was waited for above)
+			PPH_HIP(hipMemcpyAsync(p->h_bounce[b], src, span, hipMemcpyDeviceToHost, p->s_comp));
+			PPH_HIP(hipStreamSynchronize(p->s_comp));
 		}
-		const size_t n_out = p->exps.size();
-		if(p->bounce_bytes >= p->cur_size) {
-			const unsigned b = p->pf_buf;
-			if(p->pf_index != (int)index) {  // (a prefetched slice was waited for above)
-				PPH_HIP(hipMemcpyAsync(p->h_bounce[b], p->d_out + (size_t)index * p->slot_bytes, p->cur_size,
-				                       hipMemcpyDeviceToHost, p->s_comp));
-				PPH_HIP(hipStreamSynchronize(p->s_comp));
-			}
-			p->pf_index = -1;
-			if(index + 1 < n_out) {  // callers read the slices in order
-				PPH_HIP(hipMemcpyAsync(p->h_bounce[b ^ 1u], p->d_out + (size_t)(index + 1) * p->slot_bytes, p->cur_size,
-				                       hipMemcpyDeviceToHost, p->s_comp));
-				p->pf_index = (int)index + 1;
-				p->pf_buf = b ^ 1u;
-			}
-			p->copier->copy(output, p->h_bounce[b], p->cur_size);
-		} else {
-			PPH_HIP(hipMemcpy(output, p->d_out + (size_t)index * p->slot_bytes, p->cur_size, hipMemcpyDeviceToHost));
+		p->pf_index = -1;
+		if(index + 1 < p->exps.size()) {  // callers read the slices in order
+			PPH_HIP(hipMemcpyAsync(p->h_bounce[b ^ 1u], src + p->slot_bytes, span, hipMemcpyDeviceToHost, p->s_comp));
+			p->pf_index = (int)index + 1;
+			p->pf_buf = b ^ 1u;
 		}
+		host = p->h_bounce[b];
+	} else {
+		p->h_pageable.resize(span);
+		PPH_HIP(hipMemcpy(p->h_pageable.data(), src, span, hipMemcpyDeviceToHost));
+		host = p->h_pageable.data();
 	}
-	if(valid) *valid = p->h_valid[index];
+	if(p->dbg_flip_mask && p->dbg_flip_off < p->cur_size) {
+		const_cast<uint8_t*>(host)[p->dbg_flip_off] ^= p->dbg_flip_mask;
+		p->dbg_flip_mask = 0;
+	}
+	alignas(16) uint8_t h[64];
+	p->copier->copy_cksum(output, host, p->cur_size, p->ck, h, host_exp(), host_log());
+	if(valid) *valid = memcmp(h, host + ck_off, p->ck) == 0 ? 1 : 0;
 	return PPH_OK;
 }
 
@@ -1109,7 +1219,9 @@ extern "C" int pph_proc_packet_md5_update(pph_proc* p, pph_packet_md5* h) {
 }
 
 extern "C" void pph_proc_discard_output(pph_proc* p) {
-	if(p) p->processing_add = false;
+	if(!p) return;
+	p->processing_add = false;
+	p->failed = PPH_OK;
 }
 extern "C" int pph_proc_has_added(const pph_proc* p) { return p && p->processing_add ? 1 : 0; }
 extern "C" int pph_proc_is_empty(const pph_proc* p) {
@@ -1124,6 +1236,7 @@ extern "C" int pph_proc_free_processing_mem(pph_proc* p) {
 	free_outputs(p);
 	p->exps.clear();
 	p->processing_add = false;
+	p->failed = PPH_OK;
 	return PPH_OK;
 }
 extern "C" unsigned pph_proc_input_batch_size(const pph_proc* p) { return p ? p->batch : 0; }
@@ -1133,12 +1246,176 @@ extern "C" void pph_proc_set_min_input_batch_size(pph_proc* p, unsigned n) {
 }
 extern "C" int pph_proc_num_recovery_slices(const pph_proc* p) { return p ? (int)p->exps.size() : 0; }
 
+// ====================================================================== asynchronous backend
+
+namespace {
+
+void worker_main(pph_proc* p) {
+	(void)hipSetDevice(p->device);
+	std::unique_lock<std::mutex> lk(p->wmu);
+	for(;;) {
+		p->wcv.wait(lk, [&] { return p->wstop || !p->tasks.empty(); });
+		if(p->tasks.empty()) {
+			if(p->wstop) return;
+			continue;
+		}
+		pph_proc::Task t = p->tasks.front();
+		p->tasks.pop_front();
+		p->wbusy = true;
+		lk.unlock();
+		if(t.out_done) {
+			int valid = 0;
+			const int rc = pph_proc_get_output(p, t.index, t.output, &valid);
+			t.out_done(t.user, rc, valid);
+		} else {
+			int status = t.status;
+			if(t.ev) {
+				const hipError_t e = hipEventSynchronize(t.ev);
+				if(e != hipSuccess && status == PPH_OK) status = set_error(e);
+			}
+			if(t.done) t.done(t.user, status);
+		}
+		lk.lock();
+		if(t.ev) p->free_events.push_back(t.ev);
+		p->wbusy = false;
+		p->wcv.notify_all();
+	}
+}
+
+int post_task(pph_proc* p, const pph_proc::Task& t) {
+	std::lock_guard<std::mutex> lk(p->wmu);
+	if(p->wstop) return PPH_E_STATE;
+	if(!p->worker.joinable()) {
+		try {
+			p->worker = std::thread(worker_main, p);
+		} catch(...) {
+			return PPH_E_NOMEM;
+		}
+	}
+	p->tasks.push_back(t);
+	p->wcv.notify_all();
+	return PPH_OK;
+}
+
+hipEvent_t take_event(pph_proc* p) {
+	{
+		std::lock_guard<std::mutex> lk(p->wmu);
+		if(!p->free_events.empty()) {
+			hipEvent_t e = p->free_events.back();
+			p->free_events.pop_back();
+			return e;
+		}
+	}
+	hipEvent_t e = nullptr;
+	if(hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+	return e;
+}
+
+}  // namespace
+
+extern "C" unsigned pph_proc_staging_areas(const pph_proc* p) { return p ? (unsigned)p->areas.size() : 0; }
+
+extern "C" int pph_proc_stage_input(pph_proc* p, unsigned area, unsigned slot, const void* buffer, size_t size,
+                                    uint16_t input_num, const uint16_t* coeffs) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(area >= p->areas.size() || slot >= p->batch || size > p->cur_size || (size && !buffer)) return PPH_E_INVALID;
+	Area& a = p->areas[area];
+	const int rc = wait_area(p, a);
+	if(rc) return rc;
+	return stage_slot(p, a, slot, buffer, size, coeffs, input_num, false);
+}
+
+extern "C" int pph_proc_stage_dummy(pph_proc* p, unsigned area, unsigned slot, uint16_t input_num) {
+	if(!p || !p->inited) return PPH_E_STATE;
+	if(area >= p->areas.size() || slot >= p->batch) return PPH_E_INVALID;
+	Area& a = p->areas[area];
+	const int rc = wait_area(p, a);
+	if(rc) return rc;
+	return stage_slot(p, a, slot, nullptr, 0, nullptr, input_num, true);
+}
+
+extern "C" int pph_proc_fill_slot(pph_proc* p, unsigned area, unsigned slot, const void* buffer) {
+	if(!p || !p->inited || !buffer) return PPH_E_INVALID;
+	if(area >= p->areas.size() || slot >= p->batch) return PPH_E_INVALID;
+	const unsigned ca = p->cur_area, ci = p->cur_inputs;
+	p->cur_area = area;
+	p->cur_inputs = slot;
+	const int rc = pph_proc_fill_input(p, buffer);
+	p->cur_area = ca;
+	p->cur_inputs = ci;
+	return rc < 0 ? rc : PPH_OK;
+}
+
+extern "C" int pph_proc_submit(pph_proc* p, unsigned area, unsigned n_inputs, int first, pph_done_fn done,
+                               void* user) {
+	pph_proc::Task t;
+	t.done = done;
+	t.user = user;
+	if(!p || !p->inited) {
+		t.status = PPH_E_STATE;
+	} else if(area >= p->areas.size() || !n_inputs || n_inputs > p->batch) {
+		t.status = PPH_E_INVALID;
+	} else {
+		if(first) pph_proc_discard_output(p);
+		t.status = run_kernel(p, area, n_inputs);
+		if(t.status == PPH_OK) {
+			t.ev = take_event(p);
+			if(!t.ev || hipEventRecord(t.ev, p->s_comp) != hipSuccess) {
+				// completion cannot be observed through an event: wait here instead
+				(void)hipStreamSynchronize(p->s_comp);
+				if(t.ev) {
+					std::lock_guard<std::mutex> lk(p->wmu);
+					p->free_events.push_back(t.ev);
+				}
+				t.ev = nullptr;
+			}
+		}
+	}
+	// done runs exactly once: from the worker, or here when nothing can be posted
+	const int rc = p ? post_task(p, t) : t.status;
+	if(!p || rc != PPH_OK) {
+		if(t.ev) (void)hipEventSynchronize(t.ev);
+		if(done) done(user, rc != PPH_OK ? rc : t.status);
+		return rc != PPH_OK ? rc : t.status;
+	}
+	return t.status;
+}
+
+extern "C" int pph_proc_get_output_async(pph_proc* p, unsigned index, void* output, pph_output_fn done, void* user) {
+	if(!p || !p->inited || !done) return PPH_E_INVALID;
+	pph_proc::Task t;
+	t.out_done = done;
+	t.user = user;
+	t.index = index;
+	t.output = output;
+	return post_task(p, t);
+}
+
+extern "C" int pph_proc_drain(pph_proc* p) {
+	if(!p) return PPH_E_INVALID;
+	std::unique_lock<std::mutex> lk(p->wmu);
+	p->wcv.wait(lk, [&] { return p->tasks.empty() && !p->wbusy; });
+	return PPH_OK;
+}
+
 extern "C" int pph_proc_debug_corrupt_output(pph_proc* p, unsigned index, size_t offset, uint8_t mask) {
 	if(!p || !p->inited || index >= p->exps.size() || offset >= p->slot_bytes) return PPH_E_INVALID;
 	PPH_HIP(hipStreamSynchronize(p->s_comp));
 	PPH_HIP(launch_xor_byte(p->d_out + (size_t)index * p->slot_bytes + offset, mask, p->s_comp));
 	PPH_HIP(hipStreamSynchronize(p->s_comp));
-	p->verified = false;
 	p->pf_index = -1;
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_debug_corrupt_host_copy(pph_proc* p, size_t offset, uint8_t mask) {
+	if(!p || !p->inited || offset >= p->cur_size) return PPH_E_INVALID;
+	p->dbg_flip_off = offset;
+	p->dbg_flip_mask = mask;
+	return PPH_OK;
+}
+
+extern "C" int pph_proc_debug_fail_next_batch(pph_proc* p, int status) {
+	if(!p || !p->inited || status >= 0) return PPH_E_INVALID;
+	p->dbg_fail = status;
 	return PPH_OK;
 }

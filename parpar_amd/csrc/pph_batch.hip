@@ -266,7 +266,6 @@ struct pph_batch {
 	std::vector<int> form;         // PPH_FORM_PLAIN / _DENSE / _BILINEAR per job
 	void* bl_ws = nullptr;         // workspace of the bilinear jobs
 	size_t bl_ws_bytes = 0;
-	bool bl_in_graph = false;      // the captured graph holds the bilinear jobs too
 	unsigned group = 16;           // plain kernel output grouping
 	DeviceTables* tab = nullptr;
 	BatchJob* d_jobs = nullptr;
@@ -281,9 +280,8 @@ struct pph_batch {
 	bool timing = false;
 };
 
-// Bilinear jobs' mul-adds and, with verify, their checks.  Captured with the
-// rest of the batch on the hand-written fp4 path; the hipBLASLt path
-// (PPH_BILINEAR_GEMM=library) runs them after the replay, outside the graph.
+// Bilinear jobs' mul-adds and, with verify, their checks (captured with the
+// rest of the batch).
 static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s, bool verify) {
 	for(size_t j = 0; j < b->jobs.size(); j++) {
 		const pph_job& jb = b->jobs[j];
@@ -300,10 +298,8 @@ static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s, bool verify) {
 	return hipSuccess;
 }
 
-// The batch's device work.  in_graph with the library GEMMs: leave the bilinear
-// jobs' mul-adds and checks out (enqueue_bilinear runs them after the replay).
-static hipError_t enqueue_batch(pph_batch* b, hipStream_t s, bool in_graph) {
-	const bool bl_out = in_graph && !b->bl_in_graph;
+// The batch's device work.
+static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 	hipError_t e = hipSuccess;
 	if(b->n_items) {
 		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, b->d_counter, 0u);
@@ -332,12 +328,11 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s, bool in_graph) {
 		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
 		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, s);
 	}
-	if(e == hipSuccess && !bl_out) e = enqueue_bilinear(b, s, false);
+	if(e == hipSuccess) e = enqueue_bilinear(b, s, false);
 	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_b, s);
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
 		if(!jb.n_out || !jb.n_in) continue;
-		if(bl_out && b->form[j] == PPH_FORM_BILINEAR) continue;
 		e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size, ck_offset(jb.slice_size),
 		                  b->ck, jb.valid, *b->tab, s);
 	}
@@ -493,16 +488,15 @@ extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
 	hipStream_t s = (hipStream_t)stream;
 	if(!use_graph) {
 		b->timing = true;
-		hipError_t e = enqueue_batch(b, s, false);
+		hipError_t e = enqueue_batch(b, s);
 		b->timing = false;
 		if(e != hipSuccess) return set_error(e);
 		return PPH_OK;
 	}
 	if(!b->exec) {
-		b->bl_in_graph = !bl_library();
 		hipError_t e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
 		if(e != hipSuccess) return set_error(e);
-		hipError_t ee = enqueue_batch(b, b->cap, true);
+		hipError_t ee = enqueue_batch(b, b->cap);
 		e = hipStreamEndCapture(b->cap, &b->graph);
 		if(ee != hipSuccess) return set_error(ee);
 		if(e != hipSuccess) return set_error(e);
@@ -510,7 +504,6 @@ extern "C" int pph_batch_run(pph_batch* b, int use_graph, void* stream) {
 		if(e != hipSuccess) return set_error(e);
 	}
 	hipError_t e = hipGraphLaunch(b->exec, s);
-	if(e == hipSuccess && !b->bl_in_graph) e = enqueue_bilinear(b, s, true);
 	if(e != hipSuccess) return set_error(e);
 	return PPH_OK;
 }

@@ -3,6 +3,8 @@
 // returns once every part is done.  Staging copies into pinned memory run at
 // one core's copy rate otherwise, which is below the PCIe rate they feed.
 #pragma once
+#include "pph_cksum_host.h"
+
 #include <emmintrin.h>
 
 #include <algorithm>
@@ -51,6 +53,8 @@ class CopyPool {
 		uint8_t* dst;
 		const uint8_t* src;
 		size_t n;
+		CkAcc* acc;   // with a checksum: the part's Horner sum over its blocks (from zero)
+		unsigned ck;
 	};
 	std::mutex mu_;
 	std::condition_variable cv_;
@@ -67,7 +71,13 @@ class CopyPool {
 		*p = parts_[next_++];
 		return true;
 	}
-	// copy_stream has fenced the part's stores: the release publishes them to
+	static void run(const Part& p) {
+		if(p.acc)
+			ck_copy_accumulate(*p.acc, p.dst, p.src, p.n, p.ck);
+		else
+			copy_stream(p.dst, p.src, p.n);
+	}
+	// the part's stores are fenced (or ordinary): the release publishes them to
 	// the caller's acquire
 	void finish() { left_.fetch_sub(1, std::memory_order_release); }
 	void worker() {
@@ -91,7 +101,7 @@ class CopyPool {
 			Part p;
 			while(take(&p)) {
 				lk.unlock();
-				copy_stream(p.dst, p.src, p.n);
+				run(p);
 				finish();
 				lk.lock();
 			}
@@ -116,19 +126,57 @@ public:
 	CopyPool& operator=(const CopyPool&) = delete;
 
 	// One caller at a time (the backend's controlling thread).
-	void copy(void* dst, const void* src, size_t n) {
-		const size_t ways = std::min<size_t>(threads_.size() + 1, n / kPart);
-		if(ways < 2) {
-			copy_stream(dst, src, n);
+	void copy(void* dst, const void* src, size_t n) { split(dst, src, n, 0); }
+
+	// Copy n bytes and return in ck_out (ck bytes) their gf16_cksum Horner sum
+	// H = sum_k B_k * 2^(m-1-k) over the m = ceil(n/ck) blocks of ck bytes, the
+	// last one zero-padded (ck: 8, 16, 32 or 64).  Parts start on block
+	// boundaries; each sums its own blocks from zero, and the parts' sums are
+	// combined by H = sum_j H_j * 2^(blocks after part j), with the field's
+	// exp/log tables.
+	void copy_cksum(void* dst, const void* src, size_t n, unsigned ck, uint8_t* ck_out, const uint16_t* exp_tab,
+	                const uint16_t* log_tab) {
+		split(dst, src, n, ck);
+		alignas(16) uint16_t total[32] = {0};
+		const size_t m = (n + ck - 1) / ck;
+		size_t blocks_done = 0;
+		for(size_t j = 0; j < plens_.size(); j++) {
+			blocks_done += (plens_[j] + ck - 1) / ck;
+			alignas(16) uint16_t h[32];
+			ck_store(accs_[j], reinterpret_cast<uint8_t*>(h), ck);
+			ck_mul_exp_xor(total, h, ck / 2, (uint32_t)((m - blocks_done) % 65535), exp_tab, log_tab);
+		}
+		memcpy(ck_out, total, ck);
+	}
+
+private:
+	// the current copy's part lengths and, for copy_cksum, their sums (the caller's thread)
+	std::vector<size_t> plens_;
+	std::vector<CkAcc> accs_;
+
+	// Post the parts of one copy (checksummed when ck != 0), take parts on the
+	// caller's thread too, return once all are done.  Parts are multiples of 64
+	// bytes but the last, so they start on checksum block boundaries.
+	void split(void* dst, const void* src, size_t n, unsigned ck) {
+		const size_t ways = std::max<size_t>(1, std::min<size_t>(threads_.size() + 1, n / kPart));
+		const size_t step = ways < 2 ? std::max<size_t>(n, 1) : (n / ways + 63) & ~size_t(63);
+		plens_.clear();
+		for(size_t off = 0; off < n; off += step) plens_.push_back(std::min(step, n - off));
+		if(ck) {
+			accs_.resize(plens_.size());
+			for(auto& a : accs_) ck_zero(a);
+		}
+		if(plens_.size() < 2) {
+			if(n) run(Part{(uint8_t*)dst, (const uint8_t*)src, n, ck ? &accs_[0] : nullptr, ck});
 			return;
 		}
-		const size_t step = (n / ways + 63) & ~size_t(63);
 		{
 			std::lock_guard<std::mutex> lk(mu_);
 			parts_.clear();
 			next_ = 0;
-			for(size_t off = 0; off < n; off += step)
-				parts_.push_back({(uint8_t*)dst + off, (const uint8_t*)src + off, std::min(step, n - off)});
+			for(size_t j = 0; j < plens_.size(); j++)
+				parts_.push_back({(uint8_t*)dst + j * step, (const uint8_t*)src + j * step, plens_[j],
+				                  ck ? &accs_[j] : nullptr, ck});
 			left_.store(parts_.size(), std::memory_order_relaxed);
 			gen_.fetch_add(1, std::memory_order_release);
 		}
@@ -138,7 +186,7 @@ public:
 				std::lock_guard<std::mutex> lk(mu_);
 				if(!take(&p)) break;
 			}
-			copy_stream(p.dst, p.src, p.n);
+			run(p);
 			finish();
 		}
 		// the helpers' last parts are in progress: at most one part's copy time

@@ -88,10 +88,9 @@ double plain_rate(const struct ::pph_tuning& tu, unsigned n_out);
 int autotune(struct ::pph_tuning* out);
 
 // Bilinear form (pph_bilinear.hip); int status codes.
-// true when PPH_BILINEAR_GEMM=library selects the hipBLASLt GEMMs (not capturable)
-bool bl_library();
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);
+int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                      const DeviceTables& tab, hipStream_t s);
@@ -101,6 +100,8 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                 const uint16_t* wtab, const DeviceTables& tab, hipStream_t s);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
+hipError_t launch_lds_gather_ceiling(const uint16_t* exp_tab, unsigned blocks, unsigned rounds, uint32_t seed,
+                                     uint32_t* sink, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // A span [begin, end) of one file's MD5 chain, the file laid out over slots

@@ -87,7 +87,7 @@ hipError_t launch_fill_random(uint8_t* slots, size_t slot_bytes, unsigned count,
 // gf16_cksum of a slot on the device.  The slice is a sequence of CK-byte rows
 // (zero padded); the checksum is the lane-wise Horner sum
 //     ck = sum_k 2^(nb-1-k) * row_k,   nb = ceil(slice_size / CK)
-// which is what gf16_cksum_copy stores (gf16/gf16_cksum_base.h:132-146,
+// which is what gf16_cksum_copy stores (gf16/gf16_cksum_base.h:32-46,
 // including the 2^n step for zero rows after a short input).  One workgroup per
 // slot: each group of TPR threads runs Horner over a contiguous run of rows, the
 // partial sums are scaled by 2^(rows after the run) and XOR-reduced in LDS.
@@ -424,6 +424,43 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 	size_t blocks = (n + 255) / 256;
 	if(blocks > 8192) blocks = 8192;
 	hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n);
+	return hipGetLastError();
+}
+
+// Ceiling of random 16-bit lookups into the 128 KiB exp table in LDS, the
+// plain form's inner operation: one 1024-thread workgroup per CU holding the
+// table, as k_muladd_plain does; each lane gathers from 16 random byte
+// offsets, XORed each round by a wave-uniform random even offset (so every
+// round is a fresh random set of addresses with the statistics of random
+// 16-bit indices), one VALU XOR to form each address and one to keep the
+// value.  `sink` (one word per thread) keeps the results live.
+__global__ __launch_bounds__(kPlainThreads) void k_lds_gather_ceiling(const uint16_t* __restrict__ exp_tab,
+                                                                      unsigned rounds, uint32_t seed,
+                                                                      uint32_t* __restrict__ sink) {
+	__shared__ uint16_t s_exp[65536];
+	load_exp_table(s_exp, exp_tab);
+	__syncthreads();
+	uint32_t x = seed ^ (blockIdx.x * 0x9E3779B1u) ^ (threadIdx.x * 0x85EBCA77u);
+	uint32_t base[16];
+#pragma unroll
+	for(int k = 0; k < 16; k++) {
+		x ^= x << 13;
+		x ^= x >> 17;
+		x ^= x << 5;
+		base[k] = x & 0x1FFFEu;
+	}
+	uint32_t acc = 0;
+	for(unsigned i = 0; i < rounds; i++) {
+		const uint32_t u = ((i + 1) * 0x9E3779B9u ^ seed) & 0x1FFFEu;
+#pragma unroll
+		for(int k = 0; k < 16; k++)
+			acc ^= *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(s_exp) + (base[k] ^ u));
+	}
+	sink[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+hipError_t launch_lds_gather_ceiling(const uint16_t* exp_tab, unsigned blocks, unsigned rounds, uint32_t seed,
+                                     uint32_t* sink, hipStream_t s) {
+	hipLaunchKernelGGL(k_lds_gather_ceiling, dim3(blocks), dim3(kPlainThreads), 0, s, exp_tab, rounds, seed, sink);
 	return hipGetLastError();
 }
 

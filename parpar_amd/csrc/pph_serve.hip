@@ -7,7 +7,7 @@
 //   -> mul-add -> gf16_cksum check of the recovery slices
 // The mul-add is the plain form (log-domain preparation) or, for jobs where
 // the autotuned rates predict it faster, the bilinear form (data-form
-// preparation, library GEMMs in an engine-owned workspace; the bilinear
+// preparation, the fp4 MFMA GEMM in an engine-owned workspace; the bilinear
 // phases of different jobs are ordered through one event so that they take
 // the workspace one at a time).
 // Inputs come from the caller's host buffer through pinned staging chunks
@@ -159,7 +159,7 @@ struct pph_serve {
 	uint8_t* arena_mem = nullptr;
 	Arena arena;
 	std::vector<hipStream_t> streams;  // compute streams, one per running job
-	std::vector<bool> stream_busy;
+	std::vector<uint8_t> stream_busy;  // guarded by mu
 	hipStream_t hash_stream = nullptr;
 	Md5Span* h_spans[2] = {nullptr, nullptr};  // pinned
 	Md5Span* d_spans[2] = {nullptr, nullptr};
@@ -268,22 +268,23 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 // Runs on the scheduler thread.
 bool admit(pph_serve* s, Job* j) {
 	int st = -1;
-	for(unsigned k = 0; k < s->streams.size(); k++)
-		if(!s->stream_busy[k]) {
-			st = (int)k;
-			break;
-		}
-	if(st < 0) return false;
 	{
+		// the delivery thread frees streams and memory under the same lock
 		std::lock_guard<std::mutex> lk(s->mu);
+		for(unsigned k = 0; k < s->streams.size(); k++)
+			if(!s->stream_busy[k]) {
+				st = (int)k;
+				break;
+			}
+		if(st < 0) return false;
 		if(!s->arena.alloc(j->bytes_a, &j->off_a)) return false;
 		if(!s->arena.alloc(j->bytes_b, &j->off_b)) {
 			s->arena.release(j->off_a, j->bytes_a);
 			return false;
 		}
+		s->stream_busy[st] = 1;
 	}
 	j->have_a = j->have_b = true;
-	s->stream_busy[st] = true;
 	j->stream = st;
 	j->t_admit = now_s();
 	carve(s, j);
@@ -357,7 +358,12 @@ void upload_job(pph_serve* s, Job* j, Stager& st) {
 void finish_compute(pph_serve* s, Job* j) {
 	const pph_serve_job& sp = j->spec;
 	hipError_t e = hipSuccess;
-	if(j->status == PPH_OK) {
+	int status;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		status = j->status;
+	}
+	if(status == PPH_OK) {
 		if(sp.slice_md5) e = hipMemcpy2D(sp.slice_md5, 16, j->ifsc, 20, 16, j->n_in, hipMemcpyDeviceToHost);
 		if(e == hipSuccess && sp.slice_crc)
 			e = hipMemcpy2D(sp.slice_crc, 4, j->ifsc + 16, 20, 4, j->n_in, hipMemcpyDeviceToHost);
@@ -369,16 +375,16 @@ void finish_compute(pph_serve* s, Job* j) {
 		if(e == hipSuccess && sp.recovery_out && j->n_out)
 			e = hipMemcpy2D(sp.recovery_out, sp.slice_size, j->out, j->slot, sp.slice_size, j->n_out,
 			                hipMemcpyDeviceToHost);
-		if(e != hipSuccess) j->status = set_error(e);
 	} else {
 		(void)hipStreamSynchronize(s->streams[j->stream]);
 	}
 	std::lock_guard<std::mutex> lk(s->mu);
+	if(e != hipSuccess && j->status == PPH_OK) j->status = set_error(e);
 	j->t_computed = now_s();
 	j->computed = true;
 	if(j->have_b) s->arena.release(j->off_b, j->bytes_b);
 	j->have_b = false;
-	s->stream_busy[j->stream] = false;
+	s->stream_busy[j->stream] = 0;
 	s->cv.notify_all();
 }
 
@@ -389,18 +395,23 @@ void finish_job(pph_serve* s, Job* j) {
 		(void)hipStreamSynchronize(s->hash_stream);
 	}
 	for(Round& r : s->rounds) r.finals.erase(std::remove(r.finals.begin(), r.finals.end(), j), r.finals.end());
-	if(j->status == PPH_OK && sp.file_md5) {
+	int status;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		status = j->status;
+	}
+	if(status == PPH_OK && sp.file_md5) {
 		if(j->host_md5) {
 			memcpy(sp.file_md5, j->host_digest, 16);
 		} else {
 			hipError_t e = hipMemcpy(sp.file_md5, j->digest, 16, hipMemcpyDeviceToHost);
-			if(e != hipSuccess) j->status = set_error(e);
+			if(e != hipSuccess) status = set_error(e);
 		}
 	}
 	const double t = now_s();
 	pph_serve_result r{};
 	r.ticket = j->ticket;
-	r.status = j->status;
+	r.status = status;
 	r.latency_s = t - j->t_arrival;
 	r.queue_s = j->t_admit - j->t_arrival;
 	r.compute_s = j->t_computed - j->t_admit;
@@ -791,7 +802,7 @@ extern "C" int pph_serve_create(int device, const pph_serve_config* cfg, pph_ser
 	}
 	s->arena.init(c.hbm_budget);
 	s->streams.assign(c.max_compute, nullptr);
-	s->stream_busy.assign(c.max_compute, false);
+	s->stream_busy.assign(c.max_compute, 0);
 	for(auto& st : s->streams)
 		if(e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
 	if(e == hipSuccess) e = hipStreamCreateWithFlags(&s->hash_stream, hipStreamNonBlocking);

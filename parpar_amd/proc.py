@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from . import PPH_ADD_FULL, PPH_ADD_OK, PPH_ADD_OK_BUSY, FORM_AUTO, check, lib
+from . import PPH_ADD_FULL, PPH_ADD_OK, PPH_ADD_OK_BUSY, PPH_E_INVALID, PPH_E_STATE, FORM_AUTO, check, lib
 
 
 def _ptr(a):
@@ -139,12 +139,23 @@ class PAR2ProcHIP:
         check(lib().pph_proc_flush(self._p), "flush")
 
     def endInput(self):
-        check(lib().pph_proc_end_input(self._p), "endInput")
+        """Waits for every submitted batch.  A failed batch does not raise here:
+        its status is kept (last_status) and getOutput reports False."""
+        st = lib().pph_proc_end_input(self._p)
+        self.last_status = st
+        if st in (PPH_E_INVALID, PPH_E_STATE):
+            check(st, "endInput")
 
     def getOutput(self, index, output):
+        """The checksum result of the slice copied into output; False also when
+        a batch of this recovery computation failed (its status is kept in
+        last_status), as the C++ PAR2ProcHIP reports it."""
         valid = ctypes.c_int(0)
-        check(lib().pph_proc_get_output(self._p, index, _ptr(output), ctypes.byref(valid)), "getOutput")
-        return bool(valid.value)
+        st = lib().pph_proc_get_output(self._p, index, _ptr(output), ctypes.byref(valid))
+        self.last_status = st
+        if st in (PPH_E_INVALID, PPH_E_STATE):
+            check(st, "getOutput")
+        return st == 0 and bool(valid.value)
 
     def discardOutput(self):
         lib().pph_proc_discard_output(self._p)
@@ -169,6 +180,12 @@ class PAR2ProcHIP:
 
     def debug_corrupt_output(self, index, offset, mask=1):
         check(lib().pph_proc_debug_corrupt_output(self._p, index, offset, mask), "debug_corrupt_output")
+
+    def debug_corrupt_host_copy(self, offset, mask=1):
+        check(lib().pph_proc_debug_corrupt_host_copy(self._p, offset, mask), "debug_corrupt_host_copy")
+
+    def debug_fail_next_batch(self, status):
+        check(lib().pph_proc_debug_fail_next_batch(self._p, status), "debug_fail_next_batch")
 
 
 class PacketMd5:

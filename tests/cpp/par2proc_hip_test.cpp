@@ -61,6 +61,46 @@ static int run(size_t S, unsigned n, const std::vector<uint16_t>& exps, unsigned
 	return 0;
 }
 
+// A batch whose submission fails loses its inputs: every getOutput of that
+// recovery computation must then report invalid, until a new one starts.
+static int run_failed_batch() {
+	printf("case failed batch\n");
+	fflush(stdout);
+	const size_t S = 4096;
+	const unsigned n = 6;
+	std::vector<uint8_t> in((size_t)n * S);
+	for(unsigned i = 0; i < n; i++) po_prng_fill(78, 2, i, 0, &in[(size_t)i * S], S);
+	std::vector<uint16_t> exps = {0, 5};
+	PAR2ProcHIP be(0, 2);
+	be.setSliceSize(S);
+	if(!be.init(64, 2)) return fail("init");
+	if(!be.setRecoverySlices(2, exps.data())) return fail("setRecoverySlices");
+	std::vector<uint8_t> out(S);
+	for(int pass = 0; pass < 2; pass++) {
+		if(pass == 0 && be.debugFailNextBatch(PPH_E_NOMEM) != PPH_OK) return fail("fault hook");
+		for(unsigned i = 0; i < n; i++) {
+			if(be.canAdd() == PROC_ADD_FULL) be.waitForAdd();
+			be.addInput(&in[(size_t)i * S], S, (uint16_t)i, false).get();
+		}
+		be.endInput().get();
+		bool any = false;
+		for(unsigned r = 0; r < 2; r++) any = be.getOutput(r, out.data()).get() || any;
+		if(pass == 0 && any) return fail("an output of a computation with a failed batch was reported valid");
+		if(pass == 1) {
+			std::vector<uint8_t> ref(2 * S);
+			std::vector<uint16_t> nums = {0, 1, 2, 3, 4, 5};
+			std::vector<size_t> lens(n, S);
+			po_recover(n, in.data(), S, lens.data(), nums.data(), nullptr, 2, exps.data(), S, ref.data(), 0);
+			for(unsigned r = 0; r < 2; r++) {
+				if(!be.getOutput(r, out.data()).get()) return fail("output after a fresh computation");
+				if(memcmp(out.data(), &ref[(size_t)r * S], S)) return fail("output bytes after a fresh computation");
+			}
+		}
+		be.discardOutput();
+	}
+	return 0;
+}
+
 int main() {
 	po_init();
 	if(PAR2ProcHIP::getNumDevices() <= 0) return fail("no device");
@@ -76,6 +116,7 @@ int main() {
 	rc |= run(65536 + 6, 37, e2, 5, 3, 64, true);
 	rc |= run(1000, 9, e1, 64, 2, 16, true);
 	rc |= run(2, 3, e1, 1, 2, 8, false);
+	rc |= run_failed_batch();
 	if(rc) return 1;
 	printf("ALL OK\n");
 	return 0;

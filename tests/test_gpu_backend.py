@@ -269,3 +269,173 @@ def test_reference_controller_drives_par2prochip():
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "ok=1" in r.stdout
+
+
+def test_host_copy_corruption_is_reported():
+    """getOutput's flag is computed on the host over the bytes it delivers: a
+    byte flipped in host memory between the device-to-host copy and the copy to
+    the caller makes that slice invalid, while the device copy stays intact."""
+    say("host-side corruption of the copy out")
+    S = 70000
+    be = new_backend(S, 4)
+    exps = np.array([0, 3, 11], np.uint16)
+    be.setRecoverySlices(exps)
+    inp = [O.prng_bytes(26, 0, k, S) for k in range(9)]
+    for k in range(9):
+        be.addInput(inp[k], S, inputNum=k)
+    be.endInput()
+    ref = O.recover(np.stack(inp), [S] * 9, S, exps)
+    o = np.zeros(S, np.uint8)
+    assert be.getOutput(0, o) and (o == ref[0]).all()
+    be.debug_corrupt_host_copy(12345, 0x40)
+    assert not be.getOutput(1, o)
+    assert o[12345] == ref[1][12345] ^ 0x40
+    # the next copies are clean again, out of order too (no prefetch hit)
+    assert be.getOutput(2, o) and (o == ref[2]).all()
+    assert be.getOutput(1, o) and (o == ref[1]).all()
+    # the flag covers the last, partial checksum block
+    be.debug_corrupt_host_copy(S - 1, 0x01)
+    assert not be.getOutput(0, o)
+    be.close()
+
+
+@pytest.mark.parametrize("form", ["plain", "bilinear"])
+def test_failed_batch_reports_every_output_invalid(form):
+    """A batch whose submission fails loses its inputs; the backend keeps the
+    error, and every getOutput of that computation returns False (not a valid
+    flag over incomplete recovery data), until discardOutput starts afresh."""
+    say("failed batch", form)
+    S = 8192
+    be = new_backend(S, 3, form=form)
+    exps = np.array([1, 2, 9], np.uint16)
+    be.setRecoverySlices(exps)
+    inp = [O.prng_bytes(27, 0, k, S) for k in range(8)]
+    o = np.zeros(S, np.uint8)
+    for rnd in range(2):
+        if rnd == 0:
+            be.debug_fail_next_batch(parpar_amd.PPH_E_NOMEM)
+        for k in range(8):
+            if be.canAdd() == parpar_amd.PPH_ADD_FULL:
+                be.waitForAdd()
+            try:
+                be.addInput(inp[k], S, inputNum=k)
+            except parpar_amd.PPHError as e:
+                assert rnd == 0 and e.status == parpar_amd.PPH_E_NOMEM
+        be.endInput()
+        if rnd == 0:
+            assert be.last_status == parpar_amd.PPH_E_NOMEM
+            for r in range(3):
+                assert not be.getOutput(r, o)
+        else:
+            ref = O.recover(np.stack(inp), [S] * 8, S, exps)
+            for r in range(3):
+                assert be.getOutput(r, o) and (o == ref[r]).all()
+        be.discardOutput()
+    be.close()
+
+
+def test_bilinear_workspace_shortage_runs_plain():
+    """When the bilinear workspace cannot be allocated, a batch runs in the
+    plain kernel instead: the same bytes, every flag valid."""
+    say("bilinear workspace shortage")
+    lib = parpar_amd.lib()
+    S, n = 8192, 600
+    data = np.stack([O.prng_bytes(28, 0, k, S) for k in range(n)])
+    exps = np.arange(5, 5 + 70, dtype=np.uint16)
+    lib.pph_bilinear_release()
+    assert lib.pph_debug_bilinear_workspace_limit(1 << 20) == 0
+    try:
+        be = new_backend(S, 256, form="bilinear")
+        be.setRecoverySlices(exps)
+        for k in range(n):
+            be.addInput(data[k], S, inputNum=k)
+        be.endInput()
+        ref = O.recover(data, [S] * n, S, exps)
+        o = np.zeros(S, np.uint8)
+        for r in range(len(exps)):
+            assert be.getOutput(r, o) and (o == ref[r]).all(), r
+        be.close()
+    finally:
+        lib.pph_debug_bilinear_workspace_limit(0)
+
+
+def test_async_abi_explicit_areas():
+    """The asynchronous entry points (explicit area and slot, submit with a
+    completion function, getOutput on the worker thread) give the oracle's
+    bytes; completions arrive once per batch and per output."""
+    import ctypes
+    import threading
+    say("async ABI")
+    lib = parpar_amd.lib()
+    S, n, batch = 20000, 11, 4
+    exps = np.array([0, 7, 300], np.uint16)
+    be = new_backend(S, batch, areas=2)
+    be.setRecoverySlices(exps)
+    inp = [O.prng_bytes(29, 0, k, S) for k in range(n)]
+    lens = [S] * n
+    lens[-1] = 999
+    inp[-1][999:] = 0
+    assert lib.pph_proc_staging_areas(be._p) == 2
+    done = []
+    cv = threading.Condition()
+    DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int)
+    OUT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
+
+    @DONE
+    def on_batch(user, status):
+        with cv:
+            done.append(("batch", user or 0, status))
+            cv.notify_all()
+
+    @OUT
+    def on_output(user, status, valid):
+        with cv:
+            done.append(("out", user or 0, status, valid))
+            cv.notify_all()
+
+    area, slot, batches, first = 0, 0, 0, 1
+    for k in range(n):
+        buf = np.ascontiguousarray(inp[k])
+        assert lib.pph_proc_stage_input(be._p, area, slot, buf.ctypes.data, lens[k], k, None) == 0
+        slot += 1
+        if slot == batch or k == n - 1:
+            assert lib.pph_proc_submit(be._p, area, slot, first, ctypes.cast(on_batch, ctypes.c_void_p), batches) == 0
+            first, slot, batches = 0, 0, batches + 1
+            area ^= 1
+            with cv:  # the next area is free once its last batch is done
+                cv.wait_for(lambda: sum(1 for d in done if d[0] == "batch") >= batches - 1, timeout=60)
+    with cv:
+        assert cv.wait_for(lambda: sum(1 for d in done if d[0] == "batch") == batches, timeout=60)
+    assert all(d[2] == 0 for d in done)
+    outs = [np.zeros(S, np.uint8) for _ in exps]
+    for r in range(len(exps)):
+        assert lib.pph_proc_get_output_async(be._p, r, outs[r].ctypes.data, ctypes.cast(on_output, ctypes.c_void_p),
+                                             100 + r) == 0
+    assert lib.pph_proc_drain(be._p) == 0
+    got = [d for d in done if d[0] == "out"]
+    assert [d[1] for d in got] == [100, 101, 102]
+    assert all(d[2] == 0 and d[3] == 1 for d in got)
+    ref = O.recover(np.stack(inp), lens, S, exps)
+    for r in range(len(exps)):
+        assert (outs[r] == ref[r]).all(), r
+    be.close()
+
+
+@pytest.mark.parametrize("mode,args", [("hip", ["65536", "300", "20", "64"]),
+                                       ("hip", ["4098", "37", "5", "4"]),
+                                       ("split", ["65536", "120", "12", "16", "20000"]),
+                                       ("split", ["10000", "29", "3", "5", "4002"])])
+def test_reference_libuv_controller_drives_par2prochip(mode, args):
+    """ParPar's PAR2Proc in its libuv build (the form of ParPar's Node binding)
+    on a uv loop with PAR2ProcHIP's libuv form as a backend, alone or splitting
+    each slice with ParPar's own PAR2ProcCPU: every recovery slice byte-exact
+    against the oracle, every flag valid, adds refused while busy retried from
+    the progress callback."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_integration_uv")
+    if not os.path.exists(exe):
+        pytest.skip("reference controller build not shipped with this tree")
+    say("reference libuv PAR2Proc", mode, args)
+    r = subprocess.run([exe, mode] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "ALL OK" in r.stdout

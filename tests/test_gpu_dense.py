@@ -1,5 +1,5 @@
 """Dense forms' parity: the int8-MFMA bit-matrix contraction and the bilinear
-(Karatsuba, 81 int8 GEMMs) form against the oracle, the reference's golden
+(Karatsuba, 81 GEMMs on fp4 MFMA) form against the oracle, the reference's golden
 vectors and the plain kernel.  Calls go through the C ABI."""
 import hashlib
 
