@@ -14,7 +14,7 @@
  * lie into oracle/_ref/ (git-ignored).
  *
  *   ref_integration_uv --construct-only
- *   ref_integration_uv hip|split <slice> <inputs> <recovery> <batch> [cpu_share]
+ *   ref_integration_uv hip|split|cpu <slice> <inputs> <recovery> <batch> [cpu_share]
  */
 #include "controller.h"
 #include "controller_cpu.h"
@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <string>
 #include <vector>
 
 struct Run {
@@ -88,6 +89,7 @@ int main(int argc, char** argv) {
 		return 2;
 	}
 	const bool split = !strcmp(argv[1], "split");
+	const bool cpuOnly = !strcmp(argv[1], "cpu");  // PAR2ProcCPU alone: the harness itself, without a GPU
 	Run* r = new Run;
 	r->loop = loop;
 	r->S = strtoull(argv[2], nullptr, 10);
@@ -97,6 +99,7 @@ int main(int argc, char** argv) {
 	size_t cpuShare = argc > 6 ? strtoull(argv[6], nullptr, 10) : r->S / 3;
 	cpuShare &= ~(size_t)1;  // byte ranges start at even offsets (controller.cpp:42-84)
 	if(!split) cpuShare = 0;
+	if(cpuOnly) cpuShare = r->S;
 	printf("case %s S=%zu n=%u R=%u batch=%u cpu_share=%zu\n", argv[1], r->S, r->n, r->R, batch, cpuShare);
 	fflush(stdout);
 
@@ -110,10 +113,10 @@ int main(int argc, char** argv) {
 
 	// construction, as GfProc does (src/gf.cc:697-708): devices first, then the CPU
 	const size_t hipOff = cpuShare, hipSize = r->S - cpuShare;
-	PAR2ProcHIP* hip = new PAR2ProcHIP(loop, 0, 2);
+	PAR2ProcHIP* hip = cpuOnly ? nullptr : new PAR2ProcHIP(loop, 0, 2);
 	PAR2ProcCPU* cpu = cpuShare ? new PAR2ProcCPU(loop, 2) : nullptr;
 	std::vector<PAR2ProcBackendAlloc> allocs;
-	allocs.push_back({hip, hipOff, hipSize});
+	if(hip) allocs.push_back({hip, hipOff, hipSize});
 	if(cpu) allocs.push_back({cpu, 0, cpuShare});
 	if(!r->proc.init(r->S, allocs, [r](unsigned) {
 		   if(r->next < r->n) pump(r);
@@ -123,7 +126,7 @@ int main(int argc, char** argv) {
 	}
 	Galois16Mul cksumGf(Galois16Mul::default_method());
 	const unsigned ck = cksumGf.info().cksumSize;
-	if(!hip->isInitSuccess() || !hip->init(ck, batch)) {
+	if(hip && (!hip->isInitSuccess() || !hip->init(ck, batch))) {
 		printf("FAIL: PAR2ProcHIP init (status %d)\n", hip->getLastStatus());
 		return 4;
 	}
@@ -131,6 +134,8 @@ int main(int argc, char** argv) {
 		printf("FAIL: PAR2ProcCPU init\n");
 		return 5;
 	}
+	// (the CPU backend frees its method object at deinit, inside the loop)
+	const std::string method = cpu ? cpu->getMethodName() : "-";
 	if(!r->proc.setRecoverySlices(r->exps)) {
 		printf("FAIL: setRecoverySlices\n");
 		return 6;
@@ -158,8 +163,8 @@ int main(int argc, char** argv) {
 	}
 	printf("%s: %u inputs x %zu bytes -> %u recovery slices byte-exact (backend method %s, %u refused adds, "
 	       "%u batches on the GPU)\n",
-	       split ? "PAR2ProcCPU + PAR2ProcHIP" : "PAR2ProcHIP", r->n, r->S, r->R,
-	       cpu ? cpu->getMethodName() : "-", r->refusals, hip->getBatchesStarted());
+	       split ? "PAR2ProcCPU + PAR2ProcHIP" : cpuOnly ? "PAR2ProcCPU" : "PAR2ProcHIP", r->n, r->S, r->R,
+	       method.c_str(), r->refusals, hip ? hip->getBatchesStarted() : 0);
 	delete cpu;
 	delete hip;
 	delete r;

@@ -136,3 +136,36 @@ def test_bilinear_formula_matches_gf_multiply():
     rng = np.random.default_rng(81)
     for c, x in rng.integers(0, 65536, (20000, 2)):
         assert L.pph_bilinear_mul(int(c), int(x)) == R.po_mul(int(c), int(x))
+
+
+def _uv_harness():
+    import os
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "ref_integration_uv")
+    if not os.path.exists(exe):
+        pytest.skip("reference controller build not present (built where the reference sources are)")
+    return exe
+
+
+def test_libuv_form_builds_against_the_reference_controller():
+    """PAR2ProcHIP's libuv form compiles against ParPar's own gf16/controller.h
+    with USE_LIBUV and links with ParPar's PAR2Proc; without a device it
+    constructs, reports no device and closes its loop handles."""
+    import subprocess
+    r = subprocess.run([_uv_harness(), "--construct-only"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    assert "constructed" in r.stdout
+
+
+@pytest.mark.parametrize("args", [["65536", "120", "12", "16"], ["10000", "29", "3", "5"]])
+def test_libuv_harness_with_the_reference_cpu_backend(args):
+    """The uv-loop harness of the libuv integration test, run with ParPar's own
+    PAR2ProcCPU as the only backend: it drives the controller the way
+    lib/par2.js does (refused adds retried from the progress callback) and its
+    results equal the oracle's."""
+    import subprocess
+    r = subprocess.run([_uv_harness(), "cpu"] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "ALL OK" in r.stdout
