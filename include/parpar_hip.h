@@ -259,6 +259,20 @@ int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, 
                         const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes);
 int pph_bilinear_release(void);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same result as pph_muladd_bilinear, with no
+ * hipMalloc, hipFree or other allocation: the GEMM's planes and parity bits
+ * live in a static device array of PPH_NOALLOC_SCRATCH_BYTES per device (the
+ * work is chunked to fit it; every n_in <= 32768 fits, a shape that would not
+ * returns PPH_E_INVALID), and the exp and recombination tables in static
+ * device arrays.  The first call on a device finds the arrays' addresses and
+ * fills the tables (on `stream`, waited for).  Every call on a device uses
+ * the same arrays: calls that overlap on one GPU race, so serialise them
+ * (one stream, or events between streams). */
+#define PPH_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int pph_muladd_bilinear_noalloc(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                                const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
 /* Test hook: allocations of the shared bilinear workspace above `bytes` fail
  * with PPH_E_NOMEM (0 = no limit).  The backend then runs such batches in the
  * plain kernel (the same bytes). */

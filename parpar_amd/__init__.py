@@ -77,6 +77,7 @@ _SIGS = {
     "pph_proc_debug_corrupt_host_copy": (_i, [_vp, _sz, ctypes.c_uint8]),
     "pph_proc_debug_fail_next_batch": (_i, [_vp, _i]),
     "pph_debug_bilinear_workspace_limit": (_i, [_sz]),
+    "pph_muladd_bilinear_noalloc": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
     "pph_proc_staging_areas": (_u, [_vp]),
     "pph_proc_stage_input": (_i, [_vp, _u, _u, _vp, _sz, _u16, _vp]),
     "pph_proc_stage_dummy": (_i, [_vp, _u, _u, _u16]),

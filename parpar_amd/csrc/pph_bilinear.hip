@@ -200,6 +200,119 @@ int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 
 }  // namespace pph
 
+// ---------------------------------------------------------------- without allocations
+//
+// pph_muladd_bilinear_noalloc keeps every buffer its kernels use beyond the
+// caller's inputs and outputs in static device memory of this file: the
+// GEMM's planes and parity bits in `scratch` (chunked to fit), the exp table
+// in `scratch_exp` and the recombination table in `scratch_wtab` (both filled
+// by the first call on each device).  Each GPU holds its own copy of these
+// arrays for the life of the process; in host code their names are
+// stand-ins, so their device addresses come from hipGetSymbolAddress, once
+// per device, and only those addresses are used there.
+
+static __device__ __align__(256) uint8_t scratch[PPH_NOALLOC_SCRATCH_BYTES];
+static __device__ __align__(256) uint16_t scratch_exp[65536];
+static __device__ __align__(256) uint16_t scratch_wtab[pph::kBl];
+
+namespace pph {
+namespace {
+
+struct StaticScratch {
+	bool ready = false;
+	uint8_t* ws = nullptr;
+	uint16_t* exp = nullptr;
+	const uint16_t* wtab = nullptr;
+	int cus = 0;
+};
+std::mutex g_scratch_mu;
+StaticScratch g_scratch[64];
+
+// The static tables: scratch_exp[k] = 2^k in GF(2^16) (k < 65535),
+// scratch_exp[65535] = 0 (the tables' zero entry); scratch_wtab = the
+// recombination words of the bilinear formula.
+__global__ __launch_bounds__(256) void k_fill_scratch_tables() {
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if(k < kBl) {
+		constexpr BlTable kW = bl_table();
+		scratch_wtab[k] = kW.w[k];
+	}
+	if(k > 65535) return;
+	if(k == 65535) {
+		scratch_exp[k] = 0;
+		return;
+	}
+	auto mul = [](uint32_t a, uint32_t b) {
+		uint32_t r = 0;
+		for(int i = 0; i < 16; i++)
+			if((b >> i) & 1) r ^= a << i;
+		for(int i = 30; i >= 16; i--)
+			if((r >> i) & 1) r ^= kPoly << (i - 16);
+		return r;
+	};
+	uint32_t r = 1, base = 2;
+	for(uint32_t e = k; e; e >>= 1) {
+		if(e & 1) r = mul(r, base);
+		base = mul(base, base);
+	}
+	scratch_exp[k] = (uint16_t)r;
+}
+
+// Set-up on the first call on a device: the arrays' device addresses and the
+// tables (filled on the caller's stream and waited for).
+int scratch_setup(int dev, hipStream_t s, StaticScratch** out) {
+	if(dev < 0 || dev >= 64) return PPH_E_UNSUPPORTED;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	StaticScratch& st = g_scratch[dev];
+	if(!st.ready) {
+		void* ws = nullptr;
+		void* ex = nullptr;
+		void* wt = nullptr;
+		hipError_t e = hipGetSymbolAddress(&ws, HIP_SYMBOL(scratch));
+		if(e == hipSuccess) e = hipGetSymbolAddress(&ex, HIP_SYMBOL(scratch_exp));
+		if(e == hipSuccess) e = hipGetSymbolAddress(&wt, HIP_SYMBOL(scratch_wtab));
+		int cus = 0;
+		if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+		if(e != hipSuccess) return set_error(e);
+		hipLaunchKernelGGL(k_fill_scratch_tables, dim3(256), dim3(256), 0, s);
+		e = hipGetLastError();
+		if(e == hipSuccess) e = hipStreamSynchronize(s);
+		if(e != hipSuccess) return set_error(e);
+		st.ws = (uint8_t*)ws;
+		st.exp = (uint16_t*)ex;
+		st.wtab = (const uint16_t*)wt;
+		st.cus = cus;
+		st.ready = true;
+	}
+	*out = &st;
+	return PPH_OK;
+}
+
+}  // namespace
+}  // namespace pph
+
+extern "C" int pph_muladd_bilinear_noalloc(const void* in_slots, size_t slot_bytes, unsigned n_in,
+                                           const uint16_t* input_logs, const uint16_t* exps, void* out_slots,
+                                           unsigned n_out, int accumulate, void* stream) {
+	if(!n_out) return PPH_OK;
+	if(!in_slots || !out_slots || !input_logs || !exps || !n_in || n_in > 32768 || slot_bytes % kSlotAlign)
+		return PPH_E_INVALID;
+	// the planes must fit the static scratch however finely the work is chunked
+	if(blm_workspace_bytes(slot_bytes, n_in, n_out, PPH_NOALLOC_SCRATCH_BYTES) > PPH_NOALLOC_SCRATCH_BYTES)
+		return PPH_E_INVALID;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return set_error(e);
+	StaticScratch* st = nullptr;
+	int rc = scratch_setup(dev, (hipStream_t)stream, &st);
+	if(rc) return rc;
+	DeviceTables t;
+	t.exp = st->exp;
+	t.cus = st->cus;
+	return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, st->ws,
+	                   PPH_NOALLOC_SCRATCH_BYTES, st->wtab, t, (hipStream_t)stream, PPH_NOALLOC_SCRATCH_BYTES);
+}
+
 extern "C" int pph_debug_bilinear_workspace_limit(size_t bytes) {
 	g_ws_limit.store(bytes, std::memory_order_relaxed);
 	return PPH_OK;

@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 	int voff[kA + kB];
 #pragma unroll
 	for(unsigned q = 0; q < kA; q++) voff[q] = (int)(((q / (kA / 2)) * lda + 256 * (q % (kA / 2)) + tid) * 4);
-	if(RT == 256) {
+	if constexpr(RT == 256) {
 		voff[kA] = (int)(tid * 4);
 		voff[kA + 1] = (int)((ldb + tid) * 4);
 	} else {
@@ -431,8 +431,11 @@ struct BlmShape {
 	size_t cb_bytes, wb_bytes, p_bytes, total;
 };
 
-// Row tile: 128 where it pads the outputs less than 256 does.
-BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+// Row tile: 128 where it pads the outputs less than 256 does.  With cap > 0
+// the whole workspace stays within cap bytes: the coefficient block takes at
+// most a quarter of it (at least one row tile), the word chunk what is left
+// (at least one word tile); total is SIZE_MAX when even that does not fit.
+BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap) {
 	BlmShape s{};
 	s.kw_count = round_up(n_in, kBlmKStep) / 32;  // whole groups of four 64-input steps
 	s.slot_words = slot_bytes / 2;                // a multiple of 512
@@ -445,15 +448,24 @@ BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 	s.row_fast = blm_env("PPH_BILINEAR_ORDER") == 2 ? 0u : 1u;
 	const size_t per_row = kBl * s.kw_count * 4;
 	size_t rb = std::max<size_t>(s.rt, kBlmCoefBudget / per_row / s.rt * s.rt);
-	if(size_t cap = blm_env("PPH_BILINEAR_ROWS")) rb = std::min(rb, round_up(cap, s.rt));
+	if(size_t c = blm_env("PPH_BILINEAR_ROWS")) rb = std::min(rb, round_up(c, s.rt));
+	if(cap) rb = std::min(rb, std::max<size_t>(s.rt, cap / 4 / per_row / s.rt * s.rt));
 	s.r_blk = std::min(rb, round_up(n_out, s.rt));
 	const size_t per_word = kBl * s.kw_count * 4 + kBl * s.r_blk / 8;
 	size_t wc = std::max<size_t>(kBlmWordTile, kBlmChunkBudget / per_word / kBlmWordTile * kBlmWordTile);
-	if(size_t cap = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(cap, kBlmWordTile));
+	if(size_t c = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(c, kBlmWordTile));
+	s.cb_bytes = round_up(kBl * s.kw_count * s.r_blk * 4, 256);
+	if(cap) {
+		const size_t used = s.cb_bytes + 512;  // (two more regions rounded to 256)
+		wc = std::min(wc, cap > used ? (cap - used) / per_word / kBlmWordTile * kBlmWordTile : 0);
+		if(wc < kBlmWordTile) {
+			s.total = SIZE_MAX;
+			return s;
+		}
+	}
 	wc = std::min(wc, round_up(s.slot_words, kBlmWordTile));
 	const size_t chunks = (s.slot_words + wc - 1) / wc;
 	s.wc = round_up((s.slot_words + chunks - 1) / chunks, kBlmWordTile);
-	s.cb_bytes = round_up(kBl * s.kw_count * s.r_blk * 4, 256);
 	s.wb_bytes = round_up(kBl * s.kw_count * s.wc * 4, 256);
 	s.p_bytes = round_up(kBl * s.r_blk * (s.wc / 32) * 4, 256);
 	s.total = s.cb_bytes + s.wb_bytes + s.p_bytes;
@@ -462,23 +474,25 @@ BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 
 }  // namespace
 
-size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out) {
-	return blm_shape(slot_bytes, n_in, n_out).total;
+size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap) {
+	return blm_shape(slot_bytes, n_in, n_out, cap).total;
 }
 
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s) {
+                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap) {
 	if(!n_out) return PPH_OK;
-	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out);
-	if(!ws || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
+	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out, cap);
+	if(!ws || sh.total == SIZE_MAX || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
 	// the GEMM addresses one plane of a component through a buffer resource
 	if(sh.kw_count * sh.wc * 4 >= (1ull << 31) || sh.kw_count * sh.r_blk * 4 >= (1ull << 31)) return PPH_E_INVALID;
 	uint32_t* cb = static_cast<uint32_t*>(ws);
 	uint32_t* wb = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes);
 	uint32_t* par = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes + sh.wb_bytes);
-	const size_t cap = (size_t)(tab.cus > 0 ? tab.cus : 256) * 16;
-	auto blocks = [&](size_t threads) { return (unsigned)std::max<size_t>(1, std::min(cap, (threads + 255) / 256)); };
+	const size_t grid_cap = (size_t)(tab.cus > 0 ? tab.cus : 256) * 16;
+	auto blocks = [&](size_t threads) {
+		return (unsigned)std::max<size_t>(1, std::min(grid_cap, (threads + 255) / 256));
+	};
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
 		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
 		const size_t r_pad = round_up(rows, sh.rt);

@@ -94,11 +94,12 @@ int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                      const DeviceTables& tab, hipStream_t s);
-// fp4 MFMA kernel of the bilinear form (pph_bl_mfma.hip).
-size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
+// fp4 MFMA kernel of the bilinear form (pph_bl_mfma.hip).  cap > 0 chunks the
+// work so that the workspace stays within cap bytes (SIZE_MAX: it cannot).
+size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap = 0);
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s);
+                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap = 0);
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_lds_gather_ceiling(const uint16_t* exp_tab, unsigned blocks, unsigned rounds, uint32_t seed,
                                      uint32_t* sink, hipStream_t s);
