@@ -2,7 +2,9 @@
 on one MI355X.
 
 Workloads (--config):
-  1 (default)  2000 input slices x 1 MiB (uniform random, seed 1), 200 recovery
+  0 (default)  config 1's line (the headline), with configs 2, 3 and 4 run
+               after it and reported under "configs" in the same line.
+  1            2000 input slices x 1 MiB (uniform random, seed 1), 200 recovery
                slices with exponents 0..199, plain form.
   2            64 independent jobs (seed 2): inputs U[10,3000], slice size
                log-uniform in [64 KiB, 4 MiB] (multiple of 4), last input
@@ -12,31 +14,34 @@ Workloads (--config):
   3            32768 input slices x 64 KiB (seed 3), 8192 recovery slices with
                exponents 0..8191, bilinear dense form (rank-81 Karatsuba formula:
                81 GEMMs over 0/1 operands on fp4 MFMA; --form dense runs the 16x16
-               bit-matrix MFMA kernel instead), with the plain form timed on the
-               same data for the ratio.
+               bit-matrix int8 MFMA kernel instead), with the plain form timed on
+               the same data for the ratio.
   4            serving: files arrive as a Poisson process (20/s, seed 4) for a
                bounded window; file size log-uniform in [1 MiB, 8 GiB], slice
                size log-uniform in [256 KiB, 16 MiB] (multiple of 4, raised so
                that a file has at most 32768 slices), recovery count
                max(1, ceil(rho * slices)) with rho U[1%, 30%]; each file's
                slices are its PAR2 inputs.  Per job: per-slice MD5 / CRC-32,
-               file MD5, recovery slices and checksum flags back in host
-               memory.  Reports p50 / p99 latency (arrival to last output) and
-               offered vs achieved throughput.
+               file MD5 (on host threads, or on the device with
+               --md5-host-threads 0), recovery slices and checksum flags back
+               in host memory.  Reports p50 / p99 latency (arrival to last
+               output) and offered vs achieved throughput.
 
 One step is the whole device path over the batch, everything resident in HBM:
 input preparation (zero padding, gf16_cksum, log domain for the plain form),
 the mul-add, and the gf16_cksum check of every recovery slice.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 0..4] [--no-cpu]
 
 Prints one JSON line.  value = GF16 mul-add GB/s of the whole step (2 bytes
 per mul-add, MA = inputs x recovery x words).  roofline: the mul-add kernel,
-timed with HIP events on its stream -- the plain form against HBM peak
-(mul-add-equivalent bytes), the dense form against the int8 MFMA peak
-(512 int8 ops per mul-add).  cpu_baseline: the oracle (the C restatement of
-PAR2ProcCPU) with every host thread on a bounded sample of the same workload,
-compared byte for byte with the GPU's output.
+timed with HIP events on its stream -- the plain form against the random LDS
+lookup rate measured in the same run (one lookup per mul-add; its
+mul-add-equivalent rate against HBM peak beside it), the dense forms against
+the MFMA peak of their operand type.  cpu_baseline: the reference's own
+PAR2ProcCPU (a portable build of its sources, where shipped) on a bounded
+sample of the same workload, with the oracle (the C restatement) beside it,
+both compared byte for byte with the GPU's output.
 """
 import argparse
 import ctypes
@@ -47,6 +52,8 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+from parpar_amd.workloads import batched_jobs, serving_arrivals  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
@@ -61,48 +68,14 @@ MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 17.18e9 + 0.210e9,
                     ("bilinear", 32768, 1 << 16, 8192): 431.9e9 + 3.34e9}
 
 CONFIGS = {
+    # 0: the default run -- the plain workload's headline line, with the batched,
+    # dense and serving workloads measured after it and reported in the same line
+    0: dict(inputs=2000, slice=1 << 20, recovery=200, seed=1, form="plain", steps=10, warmup=2),
     1: dict(inputs=2000, slice=1 << 20, recovery=200, seed=1, form="plain", steps=10, warmup=2),
     2: dict(inputs=None, slice=None, recovery=None, seed=2, form="batched", steps=2, warmup=1),
     3: dict(inputs=32768, slice=1 << 16, recovery=8192, seed=3, form="bilinear", steps=2, warmup=1),
     4: dict(inputs=None, slice=None, recovery=None, seed=4, form="serving", steps=1, warmup=1),
 }
-
-
-def batched_jobs(seed, count=64):
-    """The batched workload's job shapes: (inputs, slice bytes, last input bytes, recovery)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for _ in range(count):
-        n = int(rng.integers(10, 3001))
-        s = int(np.exp(rng.uniform(np.log(64 << 10), np.log(4 << 20))))
-        s -= s % 4
-        last = int(rng.integers(1, s + 1))
-        r = int(rng.integers(1, 501))
-        jobs.append((n, s, last, r))
-    return jobs
-
-
-def serving_arrivals(seed, window_s, rate=20.0):
-    """Arrival times and shapes of the serving workload: (t, file bytes, slice bytes, recovery)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    t = 0.0
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t >= window_s:
-            break
-        T = int(np.exp(rng.uniform(np.log(1 << 20), np.log(8 << 30))))
-        S = int(np.exp(rng.uniform(np.log(256 << 10), np.log(16 << 20))))
-        S -= S % 4
-        if -(-T // S) > 32768:
-            S = -(-T // 32768)
-            S += (-S) % 4
-        n = -(-T // S)
-        rho = float(rng.uniform(0.01, 0.30))
-        out.append((t, T, S, max(1, int(np.ceil(rho * n)))))
-    return out
 
 
 def main_serving(args):
@@ -295,7 +268,7 @@ def main_serving(args):
                      "kernel": "whole serving pipeline (mul-add-equivalent bytes over the makespan)"},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def main_backend(args):
@@ -569,19 +542,88 @@ def main_batched(args):
         "cpu_baseline": cpu,
         "autotune": tuning.as_dict(),
     }
-    print(json.dumps(res), flush=True)
     batch.close()
+    del jobs
     L.pph_stream_destroy(stream)
+    L.pph_bilinear_release()
+    return res
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def config_args(args, config, **over):
+    import copy
+    b = copy.copy(args)
+    b.config = config
+    for k, v in CONFIGS[config].items():
+        setattr(b, k, v)
+    for k, v in over.items():
+        setattr(b, k, v)
+    return b
+
+
+def main_all(args):
+    """The default run: config 1 (plain form) is the headline; configs 2
+    (batched), 3 (bilinear and int8 bit-matrix forms against the plain one) and
+    4 (serving, file MD5 on host threads and on the device) follow, each with
+    its own CPU check, inside a time budget."""
+    t0 = time.perf_counter()
+
+    def left():
+        return args.budget_s - (time.perf_counter() - t0)
+
+    log("== config 1 (headline)")
+    res = main_kernel(config_args(args, 1, steps=args.steps, warmup=args.warmup))
+    res["config"]["config_index"] = 1
+    extra = {}
+
+    def run(name, est_s, fn):
+        if left() < est_s:
+            extra[name] = {"skipped": "time budget: %.0f s left of %.0f, about %.0f s needed" % (
+                left(), args.budget_s, est_s)}
+            log("== %s skipped (%.0f s left)" % (name, left()))
+            return
+        log("== %s" % name)
+        ts = time.perf_counter()
+        extra[name] = fn()
+        extra[name]["bench_seconds"] = round(time.perf_counter() - ts, 1)
+
+    run("2_batched", 45, lambda: main_batched(config_args(args, 2, no_cpu=args.no_cpu)))
+
+    def config3():
+        r = main_kernel(config_args(args, 3, no_cpu=args.no_cpu))
+        d = main_kernel(config_args(args, 3, form="dense", steps=1, warmup=1, no_cpu=True, no_compare=True))
+        r["dense_int8"] = {"roofline": d["roofline"], "ma_per_s": d["ma_per_s"], "ms_per_step": d["ms_per_step"],
+                           "vs_plain": round(d["ma_per_s"] / r["compare"]["plain_ma_per_s"], 3)
+                           if r.get("compare") else None}
+        return r
+
+    run("3_dense", 40, config3)
+    arrivals = serving_arrivals(4, args.serve_window)
+    run("4_serving_file_md5_host", 40,
+        lambda: main_serving(config_args(args, 4, window=args.serve_window, md5_host_threads=12, host_data=False,
+                                         no_cpu=args.no_cpu)))
+    # one wavefront's MD5 chain runs at about 90 MB/s: the largest file bounds this run
+    est = max(a[1] for a in arrivals) / 85e6 + 30
+    run("4_serving_file_md5_device", est,
+        lambda: main_serving(config_args(args, 4, window=args.serve_window, md5_host_threads=0, host_data=False,
+                                         no_cpu=args.no_cpu)))
+    res["configs"] = extra
+    res["bench_seconds"] = round(time.perf_counter() - t0, 1)
+    print(json.dumps(res), flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=0, choices=sorted(CONFIGS),
+                    help="0 (default): config 1's headline line with configs 2, 3 and 4 measured after it and "
+                         "reported in the same line; 1-4: that workload alone")
+    ap.add_argument("--budget-s", type=float, default=330.0,
+                    help="default run: a later workload is skipped (and says so) when it would end past this many "
+                         "seconds from the start")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--inputs", type=int, default=None)
@@ -594,6 +636,8 @@ def parse():
                     help="backend: kernel form (auto: bilinear for batches of 256 or more inputs with 64 or more "
                          "recovery slices, else plain)")
     ap.add_argument("--window", type=float, default=10.0, help="serving: seconds of arrivals")
+    ap.add_argument("--serve-window", type=float, default=3.0,
+                    help="default run: seconds of arrivals of its serving workload")
     ap.add_argument("--max-compute", type=int, default=8, help="serving: jobs computing at once")
     ap.add_argument("--bilinear-workspace", type=int, default=40,
                     help="serving: GiB of device workspace for the bilinear form (0: plain only)")
@@ -621,9 +665,136 @@ def parse():
     return a
 
 
+def host_cpus():
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+
 def cpu_threads():
-    t = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    return max(1, min(t, 16))
+    # the GPU host gives one GPU's job a share of 16 cores, whatever the host has
+    return max(1, min(host_cpus(), 16))
+
+
+def plain_roofline(L, ma, ker_ms, traffic, compulsory):
+    """The plain kernel against its bound: one random 16-bit lookup into the
+    128 KiB LDS exp table per mul-add, against the lookup rate measured in
+    this run by a kernel that does only such lookups (pph_lds_gather_ceiling);
+    beside it the mul-add-equivalent rate (2 B per mul-add, the unit of the
+    reference's charts) against HBM peak, and the DRAM traffic of a launch."""
+    rate = ctypes.c_double()
+    cms = ctypes.c_float()
+    parpar_amd_check(L.pph_lds_gather_ceiling(8000, ctypes.byref(rate), ctypes.byref(cms)), "lds ceiling")
+    achieved = ma / (ker_ms / 1e3) / 1e9
+    peak = rate.value / 1e9
+    eq = 2.0 * ma / (ker_ms / 1e3) / 1e9
+    roof = {"bound": "lds", "achieved": round(achieved, 1), "peak": round(peak, 1), "unit": "Glookups/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, scaled by a plain copy "
+                              "(profiles/k_muladd_plain_2000x1MiB_R200.md); not measured in this run",
+            "kernel": "k_muladd_plain", "kernel_ms": round(ker_ms, 3), "lookups_per_launch": ma,
+            "peak_source": "measured in this run: k_lds_gather_ceiling, random 16-bit lookups into the same "
+                           "128 KiB LDS table, one 1024-thread workgroup per CU (%.2f ms launch)" % cms.value,
+            "hbm_equivalent": {"achieved": round(eq, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(eq / HBM_PEAK_GBS, 4),
+                               "note": "2 B per mul-add (region bytes multiplied), not DRAM traffic"}}
+    if traffic:
+        roof["dram"] = {"GBps": round(traffic / (ker_ms / 1e3) / 1e9, 1),
+                        "frac_of_hbm": round(traffic / (ker_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "compulsory_bytes": compulsory,
+                        "note": "compulsory: every input slot read once, every recovery slot written once"}
+    return roof
+
+
+def parpar_amd_check(st, what):
+    import parpar_amd
+    return parpar_amd.check(st, what)
+
+
+def reference_container_timing():
+    """The reference CPU path timed on the full plain workload in the build
+    container (tests/golden/reference_timing.json), when present."""
+    path = os.path.join(ROOT, "tests", "golden", "reference_timing.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline_reference(args, gpu_out):
+    """The reference's own PAR2Proc + PAR2ProcCPU (portable build of its
+    sources, the method its CPUID dispatch picks on this host) on the first
+    `win` bytes of every input and recovery slice, sized to about
+    --cpu-seconds, against the GPU's bytes; the oracle (the C restatement)
+    beside it on a smaller window.  Without the reference build, the oracle
+    alone is the baseline."""
+    import numpy as np
+    import oracle as O
+
+    n, R, S = args.inputs, args.recovery, args.slice
+    exps = np.arange(R, dtype=np.uint16)
+    threads = cpu_threads()
+    inp_full = None
+
+    def inputs(win):
+        nonlocal inp_full
+        if inp_full is None or inp_full.shape[1] < win:
+            inp_full = np.stack([O.prng_bytes(args.seed, 0, i, win) for i in range(n)])
+        return np.ascontiguousarray(inp_full[:, :win])
+
+    def sized(run):
+        # calibrate on a 32 KiB column window, then size the window to about cpu_seconds
+        t, _ = run(32768)
+        win = int(args.cpu_seconds / max(t, 1e-9) * 32768)
+        win = max(32768, min(win - win % 4096, S))
+        return win, run(win)
+
+    port = None
+    if O.ref_available(portable=True):
+        info = O.ref_info(portable=True)
+
+        def run_ref(win):
+            out, valid, secs = O.ref_recover(inputs(win), [win] * n, win, exps, threads=threads, portable=True)
+            return secs, (out, valid)
+
+        win, (t, (out, valid)) = sized(run_ref)
+        ma = n * R * (win // 2)
+        cpu = {"value": round(2.0 * ma / t / 1e9, 3), "unit": "GB/s", "cores": threads, "host_cpus": host_cpus(),
+               "kind": "reference", "method": info["method"],
+               "sample": "the reference PAR2Proc + PAR2ProcCPU (its own sources, portable build, GF16_AUTO) on the "
+                         "first %d bytes of all %d input slices x %d recovery slices (%.3g mul-adds, %.1f s, "
+                         "first addInput to last getOutput)" % (win, n, R, ma, t),
+               "matches_gpu": bool((out == gpu_out[:, :win]).all()) and bool(valid.all())}
+        pwin = max(4096, min(S, win // 4) // 4096 * 4096)
+    else:
+        cpu = None
+        pwin = None
+
+    def run_port(win):
+        x = inputs(win)
+        t0 = time.perf_counter()
+        out = O.recover(x, [win] * n, win, exps, threads=threads)
+        return time.perf_counter() - t0, out
+
+    if pwin is None:
+        pwin, (t, out) = sized(run_port)
+    else:
+        t, out = run_port(pwin)
+    ma = n * R * (pwin // 2)
+    port = {"value": round(2.0 * ma / t / 1e9, 3), "unit": "GB/s", "cores": threads, "host_cpus": host_cpus(),
+            "kind": "port",
+            "sample": "the oracle (scalar split-table C restatement) on the first %d bytes of all %d input slices x "
+                      "%d recovery slices (%.3g mul-adds, %.1f s)" % (pwin, n, R, ma, t),
+            "matches_gpu": bool((out == gpu_out[:, :pwin]).all())}
+    if cpu is None:
+        cpu = port
+    else:
+        cpu["port"] = port
+    ref_here = reference_container_timing()
+    if ref_here is not None and n == 2000 and S == 1 << 20 and R == 200:
+        cpu["reference_in_build_container"] = {
+            "value": ref_here["gbps_mul_add_equivalent"], "unit": "GB/s", "cores": ref_here["threads"],
+            "method": ref_here["method"], "sample": "the whole workload, median of 3 runs (%s s)" % (
+                ref_here["seconds"],), "host": ref_here["host"]}
+    return cpu
 
 
 def cpu_baseline_window(args, gpu_out):
@@ -688,20 +859,29 @@ def main():
               "GPUs is a design note only); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.config == 2:
-        return main_batched(args)
-    if args.config == 4:
+    if args.config in (0, 4):
         if not 0 <= args.hw_queues <= 32:
             sys.exit("--hw-queues must be in 0..32")
         if args.hw_queues:
             # read by the HIP runtime when it initialises, which nothing above has done yet
             os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
-        return main_serving(args)
+    if args.config == 0:
+        return main_all(args)
+    if args.config == 2:
+        print(json.dumps(main_batched(args)), flush=True)
+        return
+    if args.config == 4:
+        print(json.dumps(main_serving(args)), flush=True)
+        return
     if args.form == "backend":
         return main_backend(args)
     if args.form == "hashes":
         return main_hashes(args)
+    print(json.dumps(main_kernel(args)), flush=True)
 
+
+def main_kernel(args):
+    """One device-resident workload in one form (plain, dense or bilinear)."""
     import numpy as np
     import parpar_amd
     from parpar_amd import proc as P
@@ -816,10 +996,7 @@ def main():
             d_rows = out.download(min(R, 16) * slot).reshape(-1, slot)[:, :S]
             compare["dense_equals_plain_rows"] = bool((d_rows == p_rows).all())
     else:
-        achieved = 2.0 * ma / (ker_avg / 1e3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(("plain", n, S, R)),
-                "kernel": "k_muladd_plain", "kernel_ms": round(ker_avg, 3), "algorithmic_bytes": 2 * ma}
+        roof = plain_roofline(L, ma, ker_avg, MEASURED_TRAFFIC.get(("plain", n, S, R)), (n + R) * slot)
 
     cpu = None
     if not args.no_cpu:
@@ -830,7 +1007,7 @@ def main():
             cpu = cpu_baseline_rows(args, got, rows)
         else:
             host = out.download(R * slot).reshape(R, slot)
-            cpu = cpu_baseline_window(args, host[:, :S])
+            cpu = cpu_baseline_reference(args, host[:, :S])
         log("cpu baseline: %s" % cpu)
 
     res = {
@@ -855,9 +1032,11 @@ def main():
     }
     if compare is not None:
         res["compare"] = compare
-    print(json.dumps(res), flush=True)
     L.pph_timer_destroy(t_ker)
     L.pph_stream_destroy(stream)
+    if dense:
+        L.pph_bilinear_release()
+    return res
 
 
 if __name__ == "__main__":

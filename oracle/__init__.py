@@ -66,6 +66,10 @@ def lib():
         L.po_prng_fill_c.restype = None
         L.po_prng_fill_c.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                      _c_u8p, ctypes.c_size_t]
+        L.po_prng_cksums.restype = None
+        L.po_prng_cksums.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t, ctypes.c_uint, _c_u8p,
+                                     ctypes.c_int]
         L.po_md5.restype = None
         L.po_md5.argtypes = [_c_u8p, ctypes.c_size_t, ctypes.c_size_t, _c_u8p]
         L.po_crc32.restype = ctypes.c_uint32
@@ -156,6 +160,19 @@ def cksum(data, slice_len, width, src_len=None):
     n = data.shape[0] if src_len is None else src_len
     out = np.zeros(width, dtype=np.uint8)
     L.po_cksum(_ptr(data), n, slice_len, width, _ptr(out))
+    return out
+
+
+def prng_cksums(seed, job, count, slice_len, lens=None, width=64, first=0, threads=0):
+    """gf16_cksum (count x width bytes) of synthetic slices first..first+count-1
+    of (seed, job), lens[k] bytes each zero-padded to slice_len."""
+    out = np.zeros((count, width), np.uint8)
+    lp = None
+    if lens is not None:
+        la = np.ascontiguousarray(lens, dtype=np.uint32)
+        assert la.size == count
+        lp = la.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    lib().po_prng_cksums(seed, job, first, count, lp, slice_len, width, _ptr(out), threads)
     return out
 
 

@@ -241,6 +241,36 @@ void po_prng_fill_c(uint64_t seed, uint32_t job, uint32_t slice, uint64_t offset
 	po_prng_fill(seed, job, slice, offset, dst, len);
 }
 
+/* gf16_cksum of synthetic slices without materialising them: slice k (of
+ * `count`, numbered first + k) holds lens[k] bytes of the generator's stream
+ * (seed, job, first + k), zero-padded to sliceLen (lens NULL: sliceLen).
+ * The composition of po_prng_fill and po_cksum, computed 64-byte block by
+ * block, slices in parallel; out[k * width .. ) receives slice k's checksum. */
+void po_prng_cksums(uint64_t seed, uint32_t job, uint32_t first, unsigned count, const uint32_t* lens,
+                    size_t sliceLen, unsigned width, uint8_t* out, int threads) {
+	po_init();
+	if(threads <= 0) threads = omp_get_max_threads();
+	#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+	for(long k = 0; k < (long)count; k++) {
+		const size_t len = lens ? lens[k] : sliceLen;
+		const unsigned lanes = width / 2;
+		uint16_t ck[32];
+		uint8_t blk[4096];
+		memset(ck, 0, sizeof(ck));
+		for(size_t pos = 0; pos < len; pos += sizeof(blk)) {
+			const size_t n = len - pos < sizeof(blk) ? len - pos : sizeof(blk);
+			po_prng_fill(seed, job, first + (uint32_t)k, pos, blk, n);
+			for(size_t b = 0; b < n; b += width) po_ck_block(ck, blk + b, n - b < width ? n - b : width, lanes);
+		}
+		if(len < sliceLen) {
+			const size_t done = (len + width - 1) / width * width;
+			if(done < sliceLen)
+				po_ck_mul(ck, po_exp((uint32_t)(((sliceLen - done + width - 1) / width) % 65535)), lanes);
+		}
+		po_ck_store(out + (size_t)k * width, ck, lanes);
+	}
+}
+
 /* ---- MD5 (RFC 1321) ---- */
 static const uint32_t po_md5_k[64] = {
 	0xd76aa478, 0xe8c7b756, 0x242070db, 0xc1bdceee, 0xf57c0faf, 0x4787c62a, 0xa8304613, 0xfd469501,

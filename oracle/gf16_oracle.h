@@ -47,6 +47,11 @@ int po_recover(unsigned nIn, const uint8_t* inputs, size_t inStride, const size_
 
 /* Synthetic inputs (oracle/po_prng.h) */
 void po_prng_fill_c(uint64_t seed, uint32_t job, uint32_t slice, uint64_t offset, uint8_t* dst, size_t len);
+/* gf16_cksum of synthetic slices first .. first+count-1 of (seed, job), lens[k]
+ * bytes each (NULL: sliceLen) zero-padded to sliceLen: po_cksum of
+ * po_prng_fill's bytes, slices in parallel (threads <= 0: all). */
+void po_prng_cksums(uint64_t seed, uint32_t job, uint32_t first, unsigned count, const uint32_t* lens,
+                    size_t sliceLen, unsigned width, uint8_t* out, int threads);
 
 /* Hashes of the per-slice path (hasher/hasher_base.h:83-198): MD5 (RFC 1321) and
  * CRC-32 (IEEE 802.3, reflected 0xEDB88320) of data zero-padded to padLen. */

@@ -207,3 +207,18 @@ def test_against_reference_build():
         ro, rv, _ = O.ref_recover(inp, lens, S, exps)
         mo = O.recover(inp, lens, S, exps)
         assert (ro == mo).all() and rv.all()
+
+
+def test_prng_cksums_is_the_checksum_of_the_generated_slices():
+    """po_prng_cksums = po_cksum over po_prng_fill's bytes, zero-padded: short,
+    empty, unaligned and full slices, numbered from an offset."""
+    S = 65536
+    lens = [1000, 4096, 3, 0, 65536, 65535, 63, 64]
+    for width in (8, 16, 32, 64):
+        got = O.prng_cksums(5, 9, len(lens), S, lens=lens, width=width, first=7)
+        for k, n in enumerate(lens):
+            data = O.prng_bytes(5, 9, 7 + k, n) if n else np.zeros(0, np.uint8)
+            assert (got[k] == O.cksum(data, S, width, src_len=n)).all(), (width, k)
+    full = O.prng_cksums(3, 0, 4, 4096)
+    for k in range(4):
+        assert (full[k] == O.cksum(O.prng_bytes(3, 0, k, 4096), 4096, 64)).all()
