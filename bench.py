@@ -541,6 +541,7 @@ def main_batched(args):
                      "algorithmic_bytes": 2 * ma},
         "cpu_baseline": cpu,
         "autotune": tuning.as_dict(),
+        "jobs": [[j.n_in, j.slice_size, j.n_out, forms[k][0]] for k, j in enumerate(jobs)],
     }
     batch.close()
     del jobs

@@ -82,8 +82,8 @@ def build_test_drivers(verbose=False):
     if not os.path.isdir(tdir):
         return
     for name in sorted(os.listdir(tdir)):
-        if not name.endswith(".cpp"):
-            continue
+        if not name.endswith(".cpp") or name.startswith("tsan_"):
+            continue  # tsan_*: built with -fsanitize=thread by tests/test_tsan.py
         exe = os.path.join(tdir, name[:-4])
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-I" + CSRC, "-I" + os.path.join(ROOT, "oracle"),
               os.path.join(tdir, name), "-o", exe, "-L" + LIBDIR, "-lparpar_hip", "-Wl,-rpath," + LIBDIR,

@@ -2,6 +2,7 @@
 // recovery backend (the IPAR2ProcBackend contract, gf16/controller.h:77-217).
 #include "pph_internal.h"
 #include "pph_copy_pool.h"
+#include "pph_task_queue.h"
 #include "../../include/parpar_hip.h"
 
 #include <algorithm>
@@ -581,11 +582,9 @@ struct pph_proc {
 		unsigned index = 0;
 		void* output = nullptr;
 	};
-	std::mutex wmu;
-	std::condition_variable wcv;
-	std::deque<Task> tasks;
+	pph::TaskQueue<Task> tasks;
+	std::mutex wmu;                       // the worker's start and the event pool
 	std::vector<hipEvent_t> free_events;
-	bool wstop = false, wbusy = false;
 	std::thread worker;
 };
 
@@ -930,15 +929,9 @@ extern "C" int pph_proc_create(int device, unsigned staging_areas, pph_proc** ou
 
 extern "C" void pph_proc_destroy(pph_proc* p) {
 	if(!p) return;
-	if(p->worker.joinable()) {
-		// the worker runs every queued task (its completion functions included) first
-		{
-			std::lock_guard<std::mutex> lk(p->wmu);
-			p->wstop = true;
-		}
-		p->wcv.notify_all();
-		p->worker.join();
-	}
+	// the worker runs every queued task (its completion functions included) first
+	p->tasks.close();
+	if(p->worker.joinable()) p->worker.join();
 	(void)hipSetDevice(p->device);
 	for(hipEvent_t e : p->free_events) (void)hipEventDestroy(e);
 	if(p->s_comp) (void)hipStreamSynchronize(p->s_comp);
@@ -1252,17 +1245,8 @@ namespace {
 
 void worker_main(pph_proc* p) {
 	(void)hipSetDevice(p->device);
-	std::unique_lock<std::mutex> lk(p->wmu);
-	for(;;) {
-		p->wcv.wait(lk, [&] { return p->wstop || !p->tasks.empty(); });
-		if(p->tasks.empty()) {
-			if(p->wstop) return;
-			continue;
-		}
-		pph_proc::Task t = p->tasks.front();
-		p->tasks.pop_front();
-		p->wbusy = true;
-		lk.unlock();
+	pph_proc::Task t;
+	while(p->tasks.pop(&t)) {
 		if(t.out_done) {
 			int valid = 0;
 			const int rc = pph_proc_get_output(p, t.index, t.output, &valid);
@@ -1274,27 +1258,27 @@ void worker_main(pph_proc* p) {
 				if(e != hipSuccess && status == PPH_OK) status = set_error(e);
 			}
 			if(t.done) t.done(t.user, status);
+			if(t.ev) {
+				std::lock_guard<std::mutex> lk(p->wmu);
+				p->free_events.push_back(t.ev);
+			}
 		}
-		lk.lock();
-		if(t.ev) p->free_events.push_back(t.ev);
-		p->wbusy = false;
-		p->wcv.notify_all();
+		p->tasks.finished();
 	}
 }
 
 int post_task(pph_proc* p, const pph_proc::Task& t) {
-	std::lock_guard<std::mutex> lk(p->wmu);
-	if(p->wstop) return PPH_E_STATE;
-	if(!p->worker.joinable()) {
-		try {
-			p->worker = std::thread(worker_main, p);
-		} catch(...) {
-			return PPH_E_NOMEM;
+	{
+		std::lock_guard<std::mutex> lk(p->wmu);
+		if(!p->worker.joinable() && !p->tasks.closed()) {
+			try {
+				p->worker = std::thread(worker_main, p);
+			} catch(...) {
+				return PPH_E_NOMEM;
+			}
 		}
 	}
-	p->tasks.push_back(t);
-	p->wcv.notify_all();
-	return PPH_OK;
+	return p->tasks.push(t) ? PPH_OK : PPH_E_STATE;
 }
 
 hipEvent_t take_event(pph_proc* p) {
@@ -1393,8 +1377,7 @@ extern "C" int pph_proc_get_output_async(pph_proc* p, unsigned index, void* outp
 
 extern "C" int pph_proc_drain(pph_proc* p) {
 	if(!p) return PPH_E_INVALID;
-	std::unique_lock<std::mutex> lk(p->wmu);
-	p->wcv.wait(lk, [&] { return p->tasks.empty() && !p->wbusy; });
+	p->tasks.wait_idle();
 	return PPH_OK;
 }
 

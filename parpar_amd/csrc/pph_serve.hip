@@ -28,6 +28,7 @@
 // its MD5 chain ends) and the log-domain inputs and recovery slices (released
 // as soon as the recovery and its checks are copied back).
 #include "pph_internal.h"
+#include "pph_md5_order.h"
 #include "pph_copy_pool.h"
 #include "pph_md5_host.h"
 #include "../../include/parpar_hip.h"
@@ -98,7 +99,6 @@ constexpr unsigned kLookBehind = 8;   // queued jobs considered behind the head
 constexpr unsigned kMaxSpans = 1024;  // files per hash round
 constexpr unsigned kMaxUploaders = 16;
 constexpr size_t kMd5ReadChunk = 4u << 20;  // pinned read-back chunk of a host MD5 thread
-constexpr double kHostMd5Rate = 0.8e9;       // bytes per second of one host MD5 chain, for the queue order
 
 struct Job {
 	pph_serve_job spec;
@@ -678,18 +678,10 @@ int hash_device_file(pph_serve* s, Job* j, Stager& st, HostMd5& h) {
 // multi-second chains of large ones.
 Job* next_host_md5(pph_serve* s) {
 	auto pick = s->host_md5.begin();
-	if(s->cfg.md5_order == 0) {
-		const double now = now_s();
-		double best = -1;
-		for(auto it = s->host_md5.begin(); it != s->host_md5.end(); ++it) {
-			const double t = (double)(*it)->spec.file_size / kHostMd5Rate + 1e-3;
-			const double ratio = (now - (*it)->t_arrival + t) / t;
-			if(ratio > best) {
-				best = ratio;
-				pick = it;
-			}
-		}
-	}
+	if(s->cfg.md5_order == 0)
+		pick = pick_highest_response_ratio(s->host_md5.begin(), s->host_md5.end(), now_s(),
+		                                   [](const Job* j) { return (uint64_t)j->spec.file_size; },
+		                                   [](const Job* j) { return j->t_arrival; });
 	Job* j = *pick;
 	s->host_md5.erase(pick);
 	return j;

@@ -19,6 +19,7 @@
 #include <node_api.h>
 
 #include "parpar_hip.h"
+#include "../csrc/pph_task_queue.h"
 
 #include <algorithm>
 #include <condition_variable>
@@ -63,10 +64,7 @@ struct GfProc {
 	napi_ref progress_cb = nullptr;
 
 	std::mutex proc_mu;  // every pph_proc call
-	std::mutex q_mu;
-	std::condition_variable q_cv;
-	std::deque<Task*> queue;
-	bool stop = false;
+	pph::TaskQueue<Task*> queue;
 	std::thread worker;
 	napi_threadsafe_function tsfn = nullptr;
 	unsigned outstanding = 0;  // posted, completion not yet run (JavaScript thread only)
@@ -75,24 +73,14 @@ struct GfProc {
 	// channel keeps the event loop alive, as a pending libuv request would.
 	void post(napi_env env, Task* t) {
 		if(outstanding++ == 0) napi_ref_threadsafe_function(env, tsfn);
-		{
-			std::lock_guard<std::mutex> lk(q_mu);
-			queue.push_back(t);
-		}
-		q_cv.notify_one();
+		queue.push(t);
 	}
 	void run() {
-		for(;;) {
-			Task* t;
-			{
-				std::unique_lock<std::mutex> lk(q_mu);
-				q_cv.wait(lk, [&] { return stop || !queue.empty(); });
-				if(queue.empty()) return;
-				t = queue.front();
-				queue.pop_front();
-			}
+		Task* t;
+		while(queue.pop(&t)) {
 			if(t->work) t->work();
 			napi_call_threadsafe_function(tsfn, t, napi_tsfn_blocking);
+			queue.finished();
 		}
 	}
 	void destroy_procs() {
@@ -157,11 +145,7 @@ void call_ref(napi_env env, napi_ref cb, std::vector<napi_value> args) {
 
 void finalize(napi_env env, void* data, void*) {
 	GfProc* g = static_cast<GfProc*>(data);
-	{
-		std::lock_guard<std::mutex> lk(g->q_mu);
-		g->stop = true;
-	}
-	g->q_cv.notify_all();
+	g->queue.close();
 	if(g->worker.joinable()) g->worker.join();
 	if(g->tsfn) napi_release_threadsafe_function(g->tsfn, napi_tsfn_abort);
 	g->destroy_procs();
