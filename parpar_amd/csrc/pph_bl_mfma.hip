@@ -303,18 +303,24 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 						                                                            0, 127, 0, 127);
 				stage((k + 4) % kBufs, pf[k]);
 				fetch(pf[k], cur, s + 4 + kDepth);
-				// LDS reads first, then MFMAs interleaved with the expansion (whose
-				// masks arrived a step ago), the LDS writes and the global loads.  The
-				// 128-row tile (five global loads and LDS writes per step) does better
-				// with its reads split around the first MFMAs.
+				// Every MFMA followed by a few instructions of the other kinds, spread
+				// evenly over the step: the expansion's VALU (whose masks arrived a step
+				// ago, 40 per step), the 8 LDS reads over the first half, the LDS writes
+				// and global loads one per four MFMAs.  Bunching the reads ahead of the
+				// MFMAs costs about 2% on a 32768-input call.  The 128-row tile (five
+				// global loads and LDS writes per step) does better with its reads split
+				// around the first MFMAs.
 				if(RT == 256) {
-					__builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // 8 LDS reads
 #pragma unroll
 					for(int q = 0; q < 16; q++) {
 						__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-						__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // 4 VALU
-						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
-						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+						if(q < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
+						if(q & 1)
+							__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // 2 VALU
+						else
+							__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // 3 VALU
+						if(q % 4 == 1) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+						if(q % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
 					}
 				} else {
 					__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
