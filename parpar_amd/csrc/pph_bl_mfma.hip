@@ -1,8 +1,8 @@
 // Bilinear form on fp4 MFMA: the 81 integer GEMMs of the rank-81 formula
 // (see pph_bilinear.hip) computed by a hand-written kernel on
 // v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 operands (0 or a power of two,
-// paired so that every product of set elements is 1; unit scales; the fp32
-// sums are exact integers up to 2^24), twice the int8 rate.
+// paired so that every product of set elements is the same power of two; the
+// fp32 sums are exact), twice the int8 rate.
 //
 // Operands are bit-packed planes, one bit per element:
 //   Cb[t][kw][r]  bit of E(c(r, 32kw + k))_t     (coefficients, outputs r)
@@ -13,7 +13,7 @@
 // outputs, and component t) stages them in LDS (one 64-input step per
 // iteration), runs 16 MFMAs per wave per step and writes the parity of each
 // sum as a bit:
-//   P[t][r][w / 32]  bit w % 32 = parity of sum_i E(c_ri)_t E(x_i[w])_t
+//   P[t][w / 32][r]  bit w % 32 = parity of sum_i E(c_ri)_t E(x_i[w])_t
 // k_blm_combine recombines the 81 parity bits of each word into the 16-bit
 // recovery word (bit-sliced over 32 words) and stores or XORs it.
 #include "pph_internal.h"
@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace pph {
 
@@ -132,65 +133,104 @@ __global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits,
 
 constexpr unsigned kBlmWordTile = 512;  // word chunks are multiples of this
 constexpr unsigned kBlmKStep = 256;     // inputs are padded to multiples of this
+constexpr unsigned kBlmMinKw = 16;      // and to at least 512 (eight steps)
+
+// One GEMM piece: the planes of one word chunk of one job.  Its items are
+// (component t, word tile, row tile), numbered from item0, row tiles varying
+// fastest (word tiles with row_fast 0).
+struct BlmPiece {
+	const uint32_t* wb;  // word planes [t][kw][wc_pad]
+	const uint32_t* cb;  // coefficient planes [t][kw][r_pad]
+	uint32_t* par;       // parity bits [t][wc_pad / 32][r_pad]
+	uint32_t steps;      // 64-input steps, a multiple of 8
+	uint32_t wc_pad;     // words, a multiple of 512
+	uint32_t r_pad;      // outputs, a multiple of the row tile
+	uint32_t w_tiles, r_tiles;
+	uint32_t item0, items;
+	uint32_t row_fast;
+};
+
+// A piece read from the table, made wave-uniform (scalar registers): the
+// table is read with vector loads, which the compiler would otherwise treat as
+// per-lane values.
+__device__ __forceinline__ BlmPiece blm_uniform(const BlmPiece& x) {
+	static_assert(sizeof(BlmPiece) % 4 == 0, "piece words");
+	uint32_t w[sizeof(BlmPiece) / 4];
+	memcpy(w, &x, sizeof(w));
+#pragma unroll
+	for(unsigned k = 0; k < sizeof(BlmPiece) / 4; k++) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+	BlmPiece y;
+	memcpy(&y, w, sizeof(w));
+	return y;
+}
 
 // Work item: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words w0 ..
 // (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
 // words).  Each wave covers 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
 // The kernel is persistent: workgroup b runs items b, b + G, b + 2G, ... (G =
-// grid size), one workgroup per CU, so the launch and the first loads are paid
-// once per workgroup rather than once per item.
-// One step = 64 inputs (two 32-input groups, one MFMA k-step); kw_count is a
-// multiple of 8, so every item is whole groups of four steps.  LDS holds the
-// bit masks ([group][row], 8 bytes per row and step, four buffers); each wave
-// expands its operands in registers, one step after reading their masks, so
-// that no expansion waits on LDS.  One barrier per step.
-// Accumulators start at 2^23: a sum n < 2^23 then sits in the low mantissa
-// bits, and its parity is bit 0 of the float's encoding.
+// grid size) of one piece (`one`, table null) or of the pieces of `table`
+// (consecutive item ranges), one workgroup per CU, so the launch is paid once
+// per workgroup rather than once per item.
+// One step = 64 inputs (two 32-input groups, one MFMA k-step); every item has
+// a multiple of four steps and at least eight.  LDS holds the bit masks
+// ([group][row], 8 bytes per row and step, four buffers); each wave expands
+// its operands in registers, one step after reading their masks, so that no
+// expansion waits on LDS.  One barrier per two steps.  The load pipeline runs
+// on across items: the last eight steps of an item fetch the first eight of
+// the next, so an item starts with its masks in LDS and registers.
+// Sums: the first step of an item adds its products to 2.0 (an inline
+// constant, so no accumulator is reset), with every product scaled by 2^-22
+// (both operand scales 2^-11), so that an accumulator holds 2 + n 2^-22
+// exactly (n < 2^22 inputs), in [2, 4) where the encoding's unit is 2^-22: the
+// parity of n is bit 0 of the encoding.
 template <unsigned RT>
-__global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ wbits, const uint32_t* __restrict__ cb,
-                                                  uint32_t* __restrict__ par, size_t kw_count, size_t wc_pad,
-                                                  size_t r_pad, unsigned w_tiles, unsigned r_tiles, unsigned items,
-                                                  unsigned row_fast) {
+__global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* __restrict__ table, unsigned items) {
 	static_assert(RT == 256 || RT == 128, "row tile");
 	constexpr unsigned WT = 65536 / RT;          // words per item
 	constexpr unsigned kA = 2 * WT / 256;        // word masks per thread and step
 	constexpr unsigned kB = 2 * RT / 256;        // coefficient masks per thread and step
-	constexpr unsigned kBufs = 4, kDepth = 4, kUnroll = 4;
-	constexpr float kBias = 8388608.0f;          // 2^23
+	constexpr unsigned kBufs = 4, kDepth = 4;
 	__shared__ __align__(16) uint32_t sA[kBufs][2][WT];
 	__shared__ __align__(16) uint32_t sB[kBufs][2][RT];
 	const unsigned G = gridDim.x, b = blockIdx.x;
 	const unsigned n_my = b < items ? (items - 1 - b) / G + 1 : 0;
 	if(!n_my) return;
-	const unsigned tiles = w_tiles * r_tiles;
-	const unsigned steps = (unsigned)(kw_count / 2);
-	const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const unsigned tid = threadIdx.x, lane = tid & 63;
+	const unsigned wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const unsigned wr = RT == 256 ? wave >> 1 : wave, wcol = RT == 256 ? wave & 1 : 0;
 	const unsigned half = lane >> 5, row32 = lane & 31;
-	// 32-bit element offsets from uniform per-item bases (a plane holds < 2^32 masks)
-	const unsigned lda = (unsigned)wc_pad, ldb = (unsigned)r_pad;
-	struct Item {
-		unsigned t, w0, r0;
+
+	// Item descriptors.  A workgroup's items only move forward through the
+	// table, so the piece is found by stepping from the last one.
+	BlmPiece P = table ? blm_uniform(table[0]) : one;
+	unsigned pc = 0;
+	struct Src {
+		__amdgpu_buffer_rsrc_t a, b, p;
+		unsigned w0, r0, steps, lda, ldb;
 	};
-	auto item = [&](unsigned j) {
+	constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit access, gfx9 family
+	auto locate = [&](unsigned j) {
 		const unsigned it = b + j * G;
-		const unsigned t = it / tiles, tile = it - t * tiles;
-		const unsigned wt = row_fast ? tile / r_tiles : tile % w_tiles;
-		const unsigned rt = row_fast ? tile % r_tiles : tile / w_tiles;
-		return Item{t, wt * WT, rt * RT};
+		if(table)
+			while(it >= P.item0 + P.items) P = blm_uniform(table[++pc]);
+		const unsigned li = it - P.item0;
+		const unsigned tiles = P.w_tiles * P.r_tiles;
+		const unsigned t = li / tiles, tile = li - t * tiles;
+		const unsigned wt = P.row_fast ? tile / P.r_tiles : tile % P.w_tiles;
+		const unsigned rt = P.row_fast ? tile % P.r_tiles : tile / P.w_tiles;
+		const size_t pa = (size_t)P.steps * P.wc_pad * 2, pb = (size_t)P.steps * P.r_pad * 2;  // 2 groups a step
+		const size_t pp = (size_t)(P.wc_pad / 32) * P.r_pad;
+		return Src{__builtin_amdgcn_make_buffer_rsrc((void*)(P.wb + t * pa), 0, (int)(pa * 4), kRsrcWord3),
+		           __builtin_amdgcn_make_buffer_rsrc((void*)(P.cb + t * pb), 0, (int)(pb * 4), kRsrcWord3),
+		           __builtin_amdgcn_make_buffer_rsrc((void*)(P.par + t * pp), 0, (int)(pp * 4), kRsrcWord3),
+		           wt * WT,
+		           rt * RT,
+		           P.steps,
+		           P.wc_pad,
+		           P.r_pad};
 	};
 
 	v16f acc[4][4];
-	auto reset = [&] {
-#pragma unroll
-		for(int i = 0; i < 4; i++)
-#pragma unroll
-			for(int j = 0; j < 4; j++)
-#pragma unroll
-				for(int v = 0; v < 16; v++) acc[i][j][v] = kBias;
-	};
-	reset();
-
 	// LDS layout [group][row]: a half-wave reads 32 consecutive masks.
 	// m[0 .. kA) = word masks (group g = index / (kA / 2), row tid + 256 (index % (kA / 2))),
 	// m[kA .. kA + kB) = coefficient masks (RT = 256: group = index; RT = 128:
@@ -223,131 +263,120 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
 		}
 	};
-	// Per item: the masks of steps 0 .. 7 come in (issued before the previous
-	// item's epilogue, so that their latency hides behind it), steps 0 .. 3 go
-	// to LDS, then the step loop.  Fetches past the item's last step repeat the
-	// last one (in bounds, never used).
 	// Loads are buffer loads: a resource per item and plane (scalar), a byte
-	// offset per thread and load that never changes, and the step's uniform
-	// byte offset in a scalar register, so the address costs no VALU work.
-	// A plane is < 2^31 bytes (the workspace chunking keeps it so); reads past
-	// the resource's range return 0.
-	struct Src {
-		__amdgpu_buffer_rsrc_t a, b;
-		unsigned w0, r0;
+	// offset per thread and load that changes only with the piece (the second
+	// group of a step is a plane row further), and the step's uniform byte
+	// offset in a scalar register, so the address costs no VALU work.  A plane
+	// is < 2^31 bytes (the workspace chunking keeps it so).
+	struct Voff {
+		int v[kA + kB];
 	};
-	constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit access, gfx9 family
-	const int plane_a = (int)(kw_count * wc_pad * 4), plane_b = (int)(kw_count * r_pad * 4);
-	auto base = [&](unsigned j) {
-		const Item x = item(j);
-		return Src{__builtin_amdgcn_make_buffer_rsrc((void*)(wbits + (size_t)x.t * kw_count * wc_pad), 0, plane_a,
-		                                             kRsrcWord3),
-		           __builtin_amdgcn_make_buffer_rsrc((void*)(cb + (size_t)x.t * kw_count * r_pad), 0, plane_b,
-		                                             kRsrcWord3),
-		           x.w0, x.r0};
+	auto voffsets = [&](const Src& f) {
+		Voff o;
+#pragma unroll
+		for(unsigned q = 0; q < kA; q++) o.v[q] = (int)(((q / (kA / 2)) * f.lda + 256 * (q % (kA / 2)) + tid) * 4);
+		if constexpr(RT == 256) {
+			o.v[kA] = (int)(tid * 4);
+			o.v[kA + 1] = (int)((f.ldb + tid) * 4);
+		} else {
+			o.v[kA] = (int)(((tid / 128) * f.ldb + tid % 128) * 4);
+		}
+		return o;
 	};
-	int voff[kA + kB];
+	auto fetch = [&](uint32_t* m, const Src& f, const Voff& o, unsigned st) {
+		const int sa = (int)((2 * st * f.lda + f.w0) * 4), sb = (int)((2 * st * f.ldb + f.r0) * 4);
 #pragma unroll
-	for(unsigned q = 0; q < kA; q++) voff[q] = (int)(((q / (kA / 2)) * lda + 256 * (q % (kA / 2)) + tid) * 4);
-	if constexpr(RT == 256) {
-		voff[kA] = (int)(tid * 4);
-		voff[kA + 1] = (int)((ldb + tid) * 4);
-	} else {
-		voff[kA] = (int)(((tid / 128) * ldb + tid % 128) * 4);
-	}
-	auto fetch = [&](uint32_t* m, const Src& f, unsigned st) {
-		const unsigned ls = st < steps ? st : steps - 1;
-		const int sa = (int)((2 * ls * lda + f.w0) * 4), sb = (int)((2 * ls * ldb + f.r0) * 4);
+		for(unsigned q = 0; q < kA; q++) m[q] = __builtin_amdgcn_raw_buffer_load_b32(f.a, o.v[q], sa, 0);
 #pragma unroll
-		for(unsigned q = 0; q < kA; q++) m[q] = __builtin_amdgcn_raw_buffer_load_b32(f.a, voff[q], sa, 0);
-#pragma unroll
-		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, voff[kA + q], sb, 0);
+		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, o.v[kA + q], sb, 0);
 	};
+	Src cur = locate(0);
+	Src nxt = locate(n_my > 1 ? 1 : 0);
+	Voff vc = voffsets(cur), vn = voffsets(nxt);
+
 	// Pipeline, step s (buffer and register-set indices static in the unrolled
 	// pass): MFMAs on the operands of step s; expansion of the masks of step
 	// s + 1 (read from LDS during step s - 1); LDS reads of the masks of step
 	// s + 2; the global masks of step s + 4 into LDS; the global fetch of step
 	// s + 8.  A barrier after every second step: a buffer is written two steps
 	// after its last read and read two steps after its write.
-	uint32_t m0[4][kA + kB], pf[kDepth][kA + kB];
-	Src cur = base(0);
+	uint32_t pf[kDepth][kA + kB];
+	v8i fa[2][4], fb[2][4];
+	uint32_t rm[2][8];
+	{
+		uint32_t m0[4][kA + kB];
 #pragma unroll
-	for(unsigned k = 0; k < 4; k++) fetch(m0[k], cur, k);
+		for(unsigned k = 0; k < 4; k++) fetch(m0[k], cur, vc, k);
 #pragma unroll
-	for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], cur, 4 + k);
-	const size_t wbk = wc_pad / 32;
-	for(unsigned cj = 0; cj < n_my; cj++) {
+		for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], cur, vc, 4 + k);
 #pragma unroll
 		for(unsigned k = 0; k < 4; k++) stage(k, m0[k]);
-		__syncthreads();
-		v8i fa[2][4], fb[2][4];
-		uint32_t rm[2][8];
-		read_masks(0, rm[0]);
-		expand(rm[0], fa[0], fb[0]);
-		read_masks(1, rm[1]);
-		__syncthreads();  // buffers 0 and 1 are written again in the first two steps
-		for(unsigned s0 = 0; s0 < steps; s0 += kUnroll) {
-			// four steps per pass: a common multiple of the four LDS buffers, the two
-			// operand and mask sets and the four prefetch sets
+	}
+	__syncthreads();
+	read_masks(0, rm[0]);
+	expand(rm[0], fa[0], fb[0]);
+	read_masks(1, rm[1]);
+	__syncthreads();  // buffers 0 and 1 are written again in the first two steps
+
+	constexpr int kScale = 116;  // e8m0 2^-11 on both operands
+	const v16f init = {2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f,
+	                   2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f};
+	// Four steps s0 .. s0 + 3: a common multiple of the four LDS buffers, the two
+	// operand and mask sets and the four prefetch sets; FIRST: s0 = 0, whose
+	// first step starts the sums.  The pass's four fetches (steps s0 + 8 ..
+	// s0 + 11) are all of this item or all of the next (step counts are
+	// multiples of four): steps st0 .. st0 + 3 of item f.
+	auto pass = [&](unsigned s0, auto first, const Src& f, const Voff& o, unsigned st0) __attribute__((always_inline)) {
 #pragma unroll
-			for(unsigned k = 0; k < kUnroll; k++) {
-				const unsigned s = s0 + k;
-				const unsigned c = k & 1, n = c ^ 1;
-				read_masks((k + 2) % kBufs, rm[c]);
-				expand(rm[n], fa[n], fb[n]);
+		for(unsigned k = 0; k < 4; k++) {
+			const unsigned c = k & 1, n = c ^ 1;
+			read_masks((k + 2) % kBufs, rm[c]);
+			expand(rm[n], fa[n], fb[n]);
 #pragma unroll
-				for(int i = 0; i < 4; i++)
+			for(int i = 0; i < 4; i++)
 #pragma unroll
-					for(int j = 0; j < 4; j++)
-						acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[c][i], fb[c][j], acc[i][j], 4, 4,
-						                                                            0, 127, 0, 127);
-				stage((k + 4) % kBufs, pf[k]);
-				fetch(pf[k], cur, s + 4 + kDepth);
-				// Every MFMA followed by a few instructions of the other kinds, spread
-				// evenly over the step: the expansion's VALU (whose masks arrived a step
-				// ago, 40 per step), the 8 LDS reads over the first half, the LDS writes
-				// and global loads one per four MFMAs.  Bunching the reads ahead of the
-				// MFMAs costs about 2% on a 32768-input call.  The 128-row tile (five
-				// global loads and LDS writes per step) does better with its reads split
-				// around the first MFMAs.
-				if(RT == 256) {
+				for(int j = 0; j < 4; j++)
+					acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+					    fa[c][i], fb[c][j], (decltype(first)::value && k == 0) ? init : acc[i][j], 4, 4, 0, kScale, 0,
+					    kScale);
+			stage((k + 4) % kBufs, pf[k]);
+			fetch(pf[k], f, o, st0 + k);
+			// Every MFMA followed by a few instructions of the other kinds, spread
+			// evenly over the step, in the counts the step has: 40 VALU (the
+			// expansion of masks that arrived a step ago), 4 LDS reads (two masks
+			// each) in the first quarter, then the LDS writes (2, or 3 for the
+			// 128-row tile) and the global loads (4, or 5).
 #pragma unroll
-					for(int q = 0; q < 16; q++) {
-						__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-						if(q < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
-						if(q & 1)
-							__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // 2 VALU
-						else
-							__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // 3 VALU
-						if(q % 4 == 1) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
-						if(q % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
-					}
-				} else {
-					__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-					__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-					__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-					__builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-#pragma unroll
-					for(int q = 0; q < 13; q++) {
-						__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-						__builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
-						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-						if(q % 3 == 0) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-					}
-				}
-				if(k & 1) __syncthreads();
+			for(int q = 0; q < 16; q++) {
+				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+				if(q < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
+				if(q & 1)
+					__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // 2 VALU
+				else
+					__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // 3 VALU
+				if(q == 4 || q == 6 || (RT == 128 && q == 8))
+					__builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+				if(q == 7 || q == 9 || q == 11 || q == 13 || (RT == 128 && q == 15))
+					__builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
 			}
+			if(k & 1) __syncthreads();
 		}
-		// the next item's first masks (the last item fetches its own again)
-		const Item x = item(cj);
-		cur = base(cj + 1 < n_my ? cj + 1 : cj);
-#pragma unroll
-		for(unsigned k = 0; k < 4; k++) fetch(m0[k], cur, k);
-#pragma unroll
-		for(unsigned k = 0; k < kDepth; k++) fetch(pf[k], cur, 4 + k);
-		// item cj done: parity bits, lane (half h, row32 = output column) holding
-		// words 8(v/4) + 4h + v%4 of each block; both half-waves hold the whole
-		// mask and store the same word (no divergent branch in the item loop).
+	};
+	for(unsigned cj = 0; cj < n_my; cj++) {
+		// the first pass fetches steps 8 .. 11: of the next item when this one
+		// has eight steps
+		if(cur.steps > 8)
+			pass(0u, std::true_type{}, cur, vc, 8u);
+		else
+			pass(0u, std::true_type{}, nxt, vn, 0u);
+		unsigned s0 = 4;
+		for(; s0 + 4 + kDepth < cur.steps; s0 += 4) pass(s0, std::false_type{}, cur, vc, s0 + 4 + kDepth);
+		for(; s0 < cur.steps; s0 += 4) pass(s0, std::false_type{}, nxt, vn, s0 + 4 + kDepth - cur.steps);
+		// item done: parity bits, lane (half h, row32 = output column) holding
+		// words m = 8(v / 4) + 4h + v % 4 of each block; gathering the low bytes
+		// of sums c, 4 + c, 8 + c, 12 + c into one dword and shifting their bit 0
+		// up by 4h + c puts word m at bit m of the block's mask.  The two
+		// half-waves hold disjoint bits of the same mask and store the same word.
 		// The sums are read with an AGPR-constrained move, which keeps the
 		// accumulators in AGPRs across the item loop (a plain VALU read inside
 		// the loop makes the allocator move them to VGPRs and spill).  A VALU
@@ -356,6 +385,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 		// 32, and every read is volatile so that none is scheduled above it
 		// (volatile asm statements keep their order).
 		asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7");
+		const int sp = (int)((((cur.w0 + 128 * wr) / 32) * cur.ldb + cur.r0 + 128 * wcol) * 4);
 #pragma unroll
 		for(int i = 0; i < 4; i++)
 #pragma unroll
@@ -363,8 +393,6 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 				uint32_t u[16];
 #pragma unroll
 				for(int v = 0; v < 16; v++) asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
-				// sum v = 4q + c goes to bit 8q + 4h + c: gather the low bytes of sums
-				// c, 4 + c, 8 + c, 12 + c into bytes 0..3, keep bit 0 of each, shift
 				uint32_t m = 0;
 #pragma unroll
 				for(int c = 0; c < 4; c++) {
@@ -372,46 +400,60 @@ __global__ __launch_bounds__(256) void k_blm_gemm(const uint32_t* __restrict__ w
 					                   __builtin_amdgcn_perm(u[12 + c], u[8 + c], 0x04000c0cu);
 					m |= (w & 0x01010101u) << (4 * half + c);
 				}
-				m |= __shfl_xor(m, 32);
-				const size_t r = x.r0 + 128 * wcol + 32 * j + row32;
-				const size_t wb = (x.w0 + 128 * wr + 32 * i) / 32;
-				par[((size_t)x.t * r_pad + r) * wbk + wb] = m;
+				const auto sw = __builtin_amdgcn_permlane32_swap(m, m, false, false);
+				m = sw[0] | sw[1];
+				__builtin_amdgcn_raw_buffer_store_b32(m, cur.p, (int)(row32 * 4), sp + (int)((i * cur.ldb + 32 * j) * 4),
+				                                      0);
 			}
-		reset();
+		cur = nxt;
+		vc = vn;
+		nxt = locate(cj + 2 < n_my ? cj + 2 : n_my - 1);
+		vn = voffsets(nxt);
 	}
 }
 
-// Recovery words from the parity bits: thread = (output row, 32-word block),
-// blocks varying fastest; the 16 result bit-slices are XORs of the parity masks
-// selected by the recombination table, then transposed to words.
+// Recovery words from the parity bits: a block of 32 output rows x 8 32-word
+// blocks, rows varying fastest within a wave (coalesced parity reads); the 16
+// result bit-slices are XORs of the parity masks selected by the recombination
+// table.  Words 2i and 2i + 1 go to one dword: a 32 x 32 bit transpose of the
+// slices' even bits (rows 0 .. 15) and odd bits shifted down (rows 16 .. 31)
+// leaves that dword in row 2i.  Each thread stores its 64 contiguous bytes as
+// four 16-byte stores (slots are 1 KiB multiples, chunks 512-word multiples).
 __global__ __launch_bounds__(256) void k_blm_combine(const uint32_t* __restrict__ par, size_t r_pad, size_t wbk,
                                                      unsigned n_rows, unsigned r0, uint8_t* __restrict__ out_slots,
                                                      size_t slot_bytes, size_t w0, int accumulate,
                                                      const uint16_t* __restrict__ wtab) {
-	const size_t total = (size_t)n_rows * wbk;
-	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const size_t r = g / wbk;
-		const size_t wb = g - r * wbk;
+	const unsigned r = blockIdx.x * 32 + (threadIdx.x & 31);
+	if(r >= n_rows) return;
+	const size_t plane = wbk * r_pad;
+	for(size_t wb = (size_t)blockIdx.y * 8 + (threadIdx.x >> 5); wb < wbk; wb += (size_t)gridDim.y * 8) {
 		uint32_t acc[16];
 #pragma unroll
 		for(int j = 0; j < 16; j++) acc[j] = 0;
+		const uint32_t* src = par + wb * r_pad + r;
+#pragma unroll 9
 		for(unsigned t = 0; t < kBl; t++) {
-			const uint32_t m = par[((size_t)t * r_pad + r) * wbk + wb];
+			const uint32_t m = src[t * plane];
 			const uint32_t w = wtab[t];
 #pragma unroll
 			for(int j = 0; j < 16; j++) acc[j] ^= m & (0u - ((w >> j) & 1u));
 		}
-		uint32_t* dst = reinterpret_cast<uint32_t*>(out_slots + (r0 + r) * slot_bytes + 2 * (w0 + 32 * wb));
+		uint32_t a[32];
 #pragma unroll
-		for(int k = 0; k < 32; k += 2) {
-			uint32_t lo = 0, hi = 0;
+		for(int j = 0; j < 16; j++) {
+			a[j] = acc[j] & 0x55555555u;
+			a[16 + j] = (acc[j] >> 1) & 0x55555555u;
+		}
+		blm_transpose32(a);
+		uint4* dst = reinterpret_cast<uint4*>(out_slots + (r0 + r) * slot_bytes + 2 * (w0 + 32 * wb));
 #pragma unroll
-			for(int j = 0; j < 16; j++) {
-				lo |= ((acc[j] >> k) & 1u) << j;
-				hi |= ((acc[j] >> (k + 1)) & 1u) << j;
+		for(int q = 0; q < 4; q++) {
+			uint4 o = make_uint4(a[8 * q], a[8 * q + 2], a[8 * q + 4], a[8 * q + 6]);
+			if(accumulate) {
+				const uint4 d = dst[q];
+				o = make_uint4(o.x ^ d.x, o.y ^ d.y, o.z ^ d.z, o.w ^ d.w);
 			}
-			const uint32_t o = lo | (hi << 16);
-			dst[k / 2] = accumulate ? (dst[k / 2] ^ o) : o;
+			dst[q] = o;
 		}
 	}
 }
@@ -443,7 +485,8 @@ struct BlmShape {
 // (at least one word tile); total is SIZE_MAX when even that does not fit.
 BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap) {
 	BlmShape s{};
-	s.kw_count = round_up(n_in, kBlmKStep) / 32;  // whole groups of four 64-input steps
+	// whole groups of four 64-input steps, at least eight steps
+	s.kw_count = std::max<size_t>(kBlmMinKw, round_up(n_in, kBlmKStep) / 32);
 	s.slot_words = slot_bytes / 2;                // a multiple of 512
 	s.rt = round_up(n_out, 128) < round_up(n_out, 256) ? 128 : 256;
 	if(size_t v = blm_env("PPH_BILINEAR_ROW_TILE")) s.rt = v == 128 ? 128 : 256;
@@ -499,6 +542,9 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 	auto blocks = [&](size_t threads) {
 		return (unsigned)std::max<size_t>(1, std::min(grid_cap, (threads + 255) / 256));
 	};
+	auto combine_grid = [](size_t rows, size_t wbk) {
+		return dim3((unsigned)((rows + 31) / 32), (unsigned)std::min<size_t>((wbk + 7) / 8, 65535));
+	};
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
 		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
 		const size_t r_pad = round_up(rows, sh.rt);
@@ -512,19 +558,18 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 			                   (const uint8_t*)in_slots, slot_bytes, n_in, w0);
 			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
 			const unsigned items = kBl * w_tiles * r_tiles;
+			const BlmPiece piece{wb, cb, par, (uint32_t)(sh.kw_count / 2), (uint32_t)wc, (uint32_t)r_pad, w_tiles, r_tiles,
+			                     0u, items, sh.row_fast};
 			const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e == hipSuccess && sh.rt == 128)
-				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc, r_pad,
-				                   w_tiles, r_tiles, items, sh.row_fast);
+				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, piece, (const BlmPiece*)nullptr, items);
 			else if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, wb, cb, par, sh.kw_count, wc, r_pad,
-				                   w_tiles, r_tiles, items, sh.row_fast);
+				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, piece, (const BlmPiece*)nullptr, items);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_combine, dim3(blocks((size_t)rows * (wc / 32))), dim3(256), 0, s, par, r_pad,
-				                   wc / 32, rows, (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0, accumulate ? 1 : 0,
-				                   wtab);
+				hipLaunchKernelGGL(k_blm_combine, combine_grid(rows, wc / 32), dim3(256), 0, s, par, r_pad, wc / 32, rows,
+				                   (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0, accumulate ? 1 : 0, wtab);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e != hipSuccess) return set_error(e);
 		}
