@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -266,6 +267,8 @@ struct pph_batch {
 	std::vector<int> form;         // PPH_FORM_PLAIN / _DENSE / _BILINEAR per job
 	void* bl_ws = nullptr;         // workspace of the bilinear jobs
 	size_t bl_ws_bytes = 0;
+	BlmBatch* bl_plan = nullptr;   // the bilinear jobs run together (cross-job launches)
+	std::vector<char> bl_alone;    // bilinear jobs outside the plan (run one by one)
 	unsigned group = 16;           // plain kernel output grouping
 	DeviceTables* tab = nullptr;
 	BatchJob* d_jobs = nullptr;
@@ -280,20 +283,19 @@ struct pph_batch {
 	bool timing = false;
 };
 
-// Bilinear jobs' mul-adds and, with verify, their checks (captured with the
-// rest of the batch).
-static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s, bool verify) {
+// Bilinear jobs' mul-adds: those of the plan in its cross-job launches, the
+// others (too large for its workspace share) one by one.
+static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s) {
+	if(b->bl_plan) {
+		const int st = blm_batch_enqueue(b->bl_plan, bilinear_wtab(), *b->tab, s);
+		if(st) return hipErrorLaunchFailure;
+	}
 	for(size_t j = 0; j < b->jobs.size(); j++) {
 		const pph_job& jb = b->jobs[j];
-		if(b->form[j] != PPH_FORM_BILINEAR || !jb.n_out) continue;
+		if(b->form[j] != PPH_FORM_BILINEAR || !jb.n_out || !b->bl_alone[j]) continue;
 		const int st = bilinear_enqueue(jb.in_slots, jb.slot_bytes, jb.n_in, jb.in_logs, jb.exps, jb.out_slots, jb.n_out,
 		                                false, b->bl_ws, b->bl_ws_bytes, *b->tab, s);
 		if(st) return hipErrorLaunchFailure;
-		if(verify) {
-			hipError_t e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size,
-			                             ck_offset(jb.slice_size), b->ck, jb.valid, *b->tab, s);
-			if(e != hipSuccess) return e;
-		}
 	}
 	return hipSuccess;
 }
@@ -328,7 +330,7 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
 		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, s);
 	}
-	if(e == hipSuccess) e = enqueue_bilinear(b, s, false);
+	if(e == hipSuccess) e = enqueue_bilinear(b, s);
 	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_b, s);
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
@@ -351,6 +353,7 @@ extern "C" void pph_batch_destroy(pph_batch* b) {
 	if(b->d_counter) (void)hipFree(b->d_counter);
 	if(b->ev_a) (void)hipEventDestroy(b->ev_a);
 	if(b->ev_b) (void)hipEventDestroy(b->ev_b);
+	blm_batch_destroy(b->bl_plan);
 	if(b->bl_ws) (void)hipFree(b->bl_ws);
 	delete b;
 }
@@ -421,17 +424,48 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		if(tb < best && bilinear_prepare(jb.slot_bytes, jb.n_in, jb.n_out) == PPH_OK) {
 			best = tb;
 			b->form[j] = PPH_FORM_BILINEAR;
-			b->bl_ws_bytes = std::max(b->bl_ws_bytes, bilinear_workspace_bytes(jb.slot_bytes, jb.n_in, jb.n_out));
 		}
 		pred[j] = best;
 		b->predicted_s += pred[j];
 	}
+	// The bilinear jobs' workspace: all of their pieces at once where that fits
+	// a budget (48 GiB, at most the free memory less 8 GiB), else waves of
+	// them; jobs whose planes would take more than a quarter of it run alone.
+	size_t budget = (size_t)48 << 30;
+	{
+		size_t fr = 0, tot = 0;
+		if(hipMemGetInfo(&fr, &tot) == hipSuccess && fr > ((size_t)8 << 30))
+			budget = std::min(budget, fr - ((size_t)8 << 30));
+		if(const char* v = getenv("PPH_BATCH_BILINEAR_WORKSPACE")) budget = (size_t)strtoull(v, nullptr, 10);
+	}
+	std::vector<BlmBatchJob> bl;
+	b->bl_alone.assign(n_jobs, 0);
+	size_t alone_ws = 0;
+	for(unsigned j = 0; j < n_jobs; j++) {
+		if(b->form[j] != PPH_FORM_BILINEAR) continue;
+		const pph_job& jb = b->jobs[j];
+		const BlmBatchJob x{jb.in_slots, jb.slot_bytes, jb.n_in, jb.in_logs, jb.exps, jb.out_slots, jb.n_out};
+		if(blm_batch_eligible(x, budget)) {
+			bl.push_back(x);
+		} else {
+			b->bl_alone[j] = 1;
+			alone_ws = std::max(alone_ws, bilinear_workspace_bytes(jb.slot_bytes, jb.n_in, jb.n_out));
+		}
+	}
+	b->bl_ws_bytes = std::max(alone_ws, blm_batch_workspace(bl.data(), (unsigned)bl.size(), budget));
 	if(b->bl_ws_bytes && hipMalloc(&b->bl_ws, b->bl_ws_bytes) != hipSuccess) {
 		(void)hipGetLastError();
 		b->bl_ws = nullptr;
 		b->bl_ws_bytes = 0;
 		pph_batch_destroy(b);
 		return PPH_E_NOMEM;
+	}
+	if(!bl.empty()) {
+		rc = blm_batch_create(bl.data(), (unsigned)bl.size(), b->bl_ws, b->bl_ws_bytes, &b->bl_plan);
+		if(rc) {
+			pph_batch_destroy(b);
+			return rc;
+		}
 	}
 
 	// work items of the plain jobs, longest job first (largest first keeps the

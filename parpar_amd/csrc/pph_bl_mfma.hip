@@ -24,7 +24,10 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <type_traits>
+#include <vector>
 
 namespace pph {
 
@@ -51,26 +54,48 @@ __device__ __forceinline__ uint4 blm_nibbles_c(uint32_t m) {
 	return make_uint4(m & 0x44444444u, m & 0x22222222u, m & 0x11111111u, (m >> 3) & 0x11111111u);
 }
 
-// Coefficient planes for outputs r0 .. r0 + n_rows (rows padded to r_pad with
-// zeros).  Thread: one output row r, one 32-input group kw; r varies fastest.
-__global__ __launch_bounds__(256) void k_blm_coef(uint32_t* __restrict__ cb, size_t kw_count, size_t r_pad,
-                                                  unsigned n_in, unsigned r0, unsigned n_rows,
-                                                  const uint16_t* __restrict__ in_logs,
-                                                  const uint16_t* __restrict__ exps,
+// A descriptor read from a table, made wave-uniform (scalar registers): the
+// tables are read with vector loads (the kernels also store to global memory),
+// which the compiler would otherwise treat as per-lane values.
+template <class T>
+__device__ __forceinline__ T blm_uniform(const T& x) {
+	static_assert(sizeof(T) % 4 == 0, "descriptor words");
+	uint32_t w[sizeof(T) / 4];
+	memcpy(w, &x, sizeof(w));
+#pragma unroll
+	for(unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+	T y;
+	memcpy(&y, w, sizeof(w));
+	return y;
+}
+
+// Coefficient planes of one job's output block: outputs r0 .. r0 + n_rows
+// (exps already offset by r0), rows padded to r_pad with zeros.
+struct BlmCoefJob {
+	uint32_t* cb;  // [t][kw_count][r_pad]
+	const uint16_t* in_logs;
+	const uint16_t* exps;
+	uint32_t kw_count, r_pad, n_in, n_rows;
+};
+
+// Coefficient planes: job `one`, or job blockIdx.y of `table`.  Thread: one
+// output row r, one 32-input group kw; r varies fastest.
+__global__ __launch_bounds__(256) void k_blm_coef(BlmCoefJob one, const BlmCoefJob* __restrict__ table,
                                                   const uint16_t* __restrict__ exp_tab) {
-	const size_t total = kw_count * r_pad;
+	const BlmCoefJob J = table ? blm_uniform(table[blockIdx.y]) : one;
+	const size_t total = (size_t)J.kw_count * J.r_pad;
 	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const size_t kw = g / r_pad;
-		const unsigned r = (unsigned)(g - kw * r_pad);
+		const size_t kw = g / J.r_pad;
+		const unsigned r = (unsigned)(g - kw * J.r_pad);
 		uint32_t s[16];
 #pragma unroll
 		for(int b = 0; b < 16; b++) s[b] = 0;
-		if(r < n_rows) {
-			const uint32_t er = exps[r0 + r];
+		if(r < J.n_rows) {
+			const uint32_t er = J.exps[r];
 			for(unsigned k = 0; k < 32; k++) {
 				const size_t i = kw * 32 + k;
-				if(i >= n_in) break;
-				uint32_t l = mod65535((uint32_t)in_logs[i] * er);
+				if(i >= J.n_in) break;
+				uint32_t l = mod65535((uint32_t)J.in_logs[i] * er);
 				const uint32_t c = exp_tab[l == kOrder ? 0u : l];
 				const unsigned p = blm_pos_c(k);
 #pragma unroll
@@ -80,7 +105,7 @@ __global__ __launch_bounds__(256) void k_blm_coef(uint32_t* __restrict__ cb, siz
 		uint32_t e[kBl];
 		bl_expand_slices(s, e);
 #pragma unroll
-		for(unsigned t = 0; t < kBl; t++) cb[((size_t)t * kw_count + kw) * r_pad + r] = e[t];
+		for(unsigned t = 0; t < kBl; t++) J.cb[((size_t)t * J.kw_count + kw) * J.r_pad + r] = e[t];
 	}
 }
 
@@ -99,17 +124,24 @@ __device__ __forceinline__ void blm_transpose32(uint32_t* a) {
 	}
 }
 
-// Word planes for words [w0, w0 + wc) of every input slot (rows padded to
-// wc_pad with zeros).  Thread: two adjacent words, one 32-input group; words
-// vary fastest so each of the 32 slot reads (4 bytes) is coalesced.  Row j of
-// the 32 x 32 bit matrix is input 8 (j % 4) + j / 4 (the packed position
-// order), so after the transpose row b < 16 is bit slice b of the first word
-// and row 16 + b that of the second.
-__global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits, size_t kw_count, size_t wc_pad,
-                                                   size_t wc, const uint8_t* __restrict__ in_slots, size_t slot_bytes,
-                                                   unsigned n_in, size_t w0) {
-	const size_t pairs = wc_pad / 2;
-	const size_t total = kw_count * pairs;
+// Word planes of one word chunk of one job: words w0 .. w0 + wc of every input
+// slot (`in` already offset to word w0), wc a multiple of 512.
+struct BlmWordsPiece {
+	uint32_t* wb;  // [t][kw_count][wc]
+	const uint8_t* in;
+	uint64_t slot_bytes;
+	uint32_t kw_count, wc, n_in, pad;
+};
+
+// Word planes: piece `one`, or piece blockIdx.y of `table`.  Thread: two
+// adjacent words, one 32-input group; words vary fastest so each of the 32
+// slot reads (4 bytes) is coalesced.  Row j of the 32 x 32 bit matrix is input
+// 8 (j % 4) + j / 4 (the packed position order), so after the transpose row
+// b < 16 is bit slice b of the first word and row 16 + b that of the second.
+__global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmWordsPiece* __restrict__ table) {
+	const BlmWordsPiece P = table ? blm_uniform(table[blockIdx.y]) : one;
+	const size_t pairs = P.wc / 2;
+	const size_t total = (size_t)P.kw_count * pairs;
 	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
 		const size_t kw = g / pairs;
 		const size_t wl = 2 * (g - kw * pairs);
@@ -117,9 +149,7 @@ __global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits,
 #pragma unroll
 		for(unsigned j = 0; j < 32; j++) {
 			const size_t i = kw * 32 + 8 * (j % 4) + j / 4;
-			a[j] = (wl < wc && i < n_in)
-			           ? reinterpret_cast<const uint32_t*>(in_slots + i * slot_bytes)[(w0 + wl) / 2]
-			           : 0u;
+			a[j] = i < P.n_in ? reinterpret_cast<const uint32_t*>(P.in + i * P.slot_bytes)[wl / 2] : 0u;
 		}
 		blm_transpose32(a);
 		uint32_t e0[kBl], e1[kBl];
@@ -127,7 +157,7 @@ __global__ __launch_bounds__(256) void k_blm_words(uint32_t* __restrict__ wbits,
 		bl_expand_slices(a + 16, e1);
 #pragma unroll
 		for(unsigned t = 0; t < kBl; t++)
-			*reinterpret_cast<uint2*>(wbits + ((size_t)t * kw_count + kw) * wc_pad + wl) = make_uint2(e0[t], e1[t]);
+			*reinterpret_cast<uint2*>(P.wb + ((size_t)t * P.kw_count + kw) * P.wc + wl) = make_uint2(e0[t], e1[t]);
 	}
 }
 
@@ -149,20 +179,6 @@ struct BlmPiece {
 	uint32_t item0, items;
 	uint32_t row_fast;
 };
-
-// A piece read from the table, made wave-uniform (scalar registers): the
-// table is read with vector loads, which the compiler would otherwise treat as
-// per-lane values.
-__device__ __forceinline__ BlmPiece blm_uniform(const BlmPiece& x) {
-	static_assert(sizeof(BlmPiece) % 4 == 0, "piece words");
-	uint32_t w[sizeof(BlmPiece) / 4];
-	memcpy(w, &x, sizeof(w));
-#pragma unroll
-	for(unsigned k = 0; k < sizeof(BlmPiece) / 4; k++) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
-	BlmPiece y;
-	memcpy(&y, w, sizeof(w));
-	return y;
-}
 
 // Work item: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words w0 ..
 // (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
@@ -412,25 +428,34 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	}
 }
 
-// Recovery words from the parity bits: a block of 32 output rows x 8 32-word
-// blocks, rows varying fastest within a wave (coalesced parity reads); the 16
-// result bit-slices are XORs of the parity masks selected by the recombination
-// table.  Words 2i and 2i + 1 go to one dword: a 32 x 32 bit transpose of the
-// slices' even bits (rows 0 .. 15) and odd bits shifted down (rows 16 .. 31)
-// leaves that dword in row 2i.  Each thread stores its 64 contiguous bytes as
-// four 16-byte stores (slots are 1 KiB multiples, chunks 512-word multiples).
-__global__ __launch_bounds__(256) void k_blm_combine(const uint32_t* __restrict__ par, size_t r_pad, size_t wbk,
-                                                     unsigned n_rows, unsigned r0, uint8_t* __restrict__ out_slots,
-                                                     size_t slot_bytes, size_t w0, int accumulate,
+// Recombination of one piece: outputs r0 .. r0 + n_rows, words w0 .. w0 +
+// 32 wbk (`out` already offset to row r0 and word w0).
+struct BlmCombinePiece {
+	const uint32_t* par;  // [t][wbk][r_pad]
+	uint8_t* out;
+	uint64_t slot_bytes;
+	uint32_t r_pad, wbk, n_rows, accumulate;
+};
+
+// Recovery words from the parity bits of piece `one`, or of piece blockIdx.z
+// of `table`: a block of 32 output rows x 8 32-word blocks, rows varying
+// fastest within a wave (coalesced parity reads); the 16 result bit-slices are
+// XORs of the parity masks selected by the recombination table.  Words 2i and
+// 2i + 1 go to one dword: a 32 x 32 bit transpose of the slices' even bits
+// (rows 0 .. 15) and odd bits shifted down (rows 16 .. 31) leaves that dword
+// in row 2i.  Each thread stores its 64 contiguous bytes as four 16-byte
+// stores (slots are 1 KiB multiples, chunks 512-word multiples).
+__global__ __launch_bounds__(256) void k_blm_combine(BlmCombinePiece one, const BlmCombinePiece* __restrict__ table,
                                                      const uint16_t* __restrict__ wtab) {
+	const BlmCombinePiece P = table ? blm_uniform(table[blockIdx.z]) : one;
 	const unsigned r = blockIdx.x * 32 + (threadIdx.x & 31);
-	if(r >= n_rows) return;
-	const size_t plane = wbk * r_pad;
-	for(size_t wb = (size_t)blockIdx.y * 8 + (threadIdx.x >> 5); wb < wbk; wb += (size_t)gridDim.y * 8) {
+	if(r >= P.n_rows) return;
+	const size_t plane = (size_t)P.wbk * P.r_pad;
+	for(size_t wb = (size_t)blockIdx.y * 8 + (threadIdx.x >> 5); wb < P.wbk; wb += (size_t)gridDim.y * 8) {
 		uint32_t acc[16];
 #pragma unroll
 		for(int j = 0; j < 16; j++) acc[j] = 0;
-		const uint32_t* src = par + wb * r_pad + r;
+		const uint32_t* src = P.par + wb * P.r_pad + r;
 #pragma unroll 9
 		for(unsigned t = 0; t < kBl; t++) {
 			const uint32_t m = src[t * plane];
@@ -445,11 +470,11 @@ __global__ __launch_bounds__(256) void k_blm_combine(const uint32_t* __restrict_
 			a[16 + j] = (acc[j] >> 1) & 0x55555555u;
 		}
 		blm_transpose32(a);
-		uint4* dst = reinterpret_cast<uint4*>(out_slots + (r0 + r) * slot_bytes + 2 * (w0 + 32 * wb));
+		uint4* dst = reinterpret_cast<uint4*>(P.out + r * P.slot_bytes + 64 * wb);
 #pragma unroll
 		for(int q = 0; q < 4; q++) {
 			uint4 o = make_uint4(a[8 * q], a[8 * q + 2], a[8 * q + 4], a[8 * q + 6]);
-			if(accumulate) {
+			if(P.accumulate) {
 				const uint4 d = dst[q];
 				o = make_uint4(o.x ^ d.x, o.y ^ d.y, o.z ^ d.z, o.w ^ d.w);
 			}
@@ -527,54 +552,290 @@ size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, siz
 	return blm_shape(slot_bytes, n_in, n_out, cap).total;
 }
 
+namespace {
+
+unsigned blm_blocks(size_t threads, const DeviceTables& tab) {
+	const size_t cap = (size_t)(tab.cus > 0 ? tab.cus : 256) * 16;
+	return (unsigned)std::max<size_t>(1, std::min(cap, (threads + 255) / 256));
+}
+
+// a plane of one component is addressed through a buffer resource
+bool blm_planes_fit(size_t kw_count, size_t wc, size_t r_pad) {
+	const size_t lim = 1ull << 31;
+	return kw_count * wc * 4 < lim && kw_count * r_pad * 4 < lim && (wc / 32) * r_pad * 4 < lim;
+}
+
+hipError_t blm_launch_gemm(unsigned rt, const BlmPiece& one, const BlmPiece* table, unsigned items,
+                           const DeviceTables& tab, hipStream_t s) {
+	if(!items) return hipSuccess;
+	const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
+	if(rt == 128)
+		hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, one, table, items);
+	else
+		hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, one, table, items);
+	return hipGetLastError();
+}
+
+dim3 blm_combine_grid(size_t rows, size_t wbk, unsigned pieces) {
+	return dim3((unsigned)((rows + 31) / 32), (unsigned)std::min<size_t>((wbk + 7) / 8, 4096), pieces);
+}
+
+}  // namespace
+
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                 const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap) {
 	if(!n_out) return PPH_OK;
 	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out, cap);
 	if(!ws || sh.total == SIZE_MAX || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
-	// the GEMM addresses one plane of a component through a buffer resource
-	if(sh.kw_count * sh.wc * 4 >= (1ull << 31) || sh.kw_count * sh.r_blk * 4 >= (1ull << 31)) return PPH_E_INVALID;
+	if(!blm_planes_fit(sh.kw_count, sh.wc, round_up(sh.r_blk, sh.rt))) return PPH_E_INVALID;
 	uint32_t* cb = static_cast<uint32_t*>(ws);
 	uint32_t* wb = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes);
 	uint32_t* par = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes + sh.wb_bytes);
-	const size_t grid_cap = (size_t)(tab.cus > 0 ? tab.cus : 256) * 16;
-	auto blocks = [&](size_t threads) {
-		return (unsigned)std::max<size_t>(1, std::min(grid_cap, (threads + 255) / 256));
-	};
-	auto combine_grid = [](size_t rows, size_t wbk) {
-		return dim3((unsigned)((rows + 31) / 32), (unsigned)std::min<size_t>((wbk + 7) / 8, 65535));
-	};
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
 		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
 		const size_t r_pad = round_up(rows, sh.rt);
-		hipLaunchKernelGGL(k_blm_coef, dim3(blocks(sh.kw_count * r_pad)), dim3(256), 0, s, cb, sh.kw_count, r_pad, n_in,
-		                   (unsigned)r0, rows, input_logs, exps, tab.exp);
+		const BlmCoefJob cj{cb, input_logs, exps + r0, (uint32_t)sh.kw_count, (uint32_t)r_pad, n_in, rows};
+		hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(sh.kw_count * r_pad, tab)), dim3(256), 0, s, cj,
+		                   (const BlmCoefJob*)nullptr, tab.exp);
 		hipError_t e = hipGetLastError();
 		if(e != hipSuccess) return set_error(e);
 		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
 			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 512
-			hipLaunchKernelGGL(k_blm_words, dim3(blocks(sh.kw_count * wc / 2)), dim3(256), 0, s, wb, sh.kw_count, wc, wc,
-			                   (const uint8_t*)in_slots, slot_bytes, n_in, w0);
+			const BlmWordsPiece wp{wb, (const uint8_t*)in_slots + 2 * w0, slot_bytes, (uint32_t)sh.kw_count,
+			                       (uint32_t)wc, n_in, 0u};
+			hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(sh.kw_count * wc / 2, tab)), dim3(256), 0, s, wp,
+			                   (const BlmWordsPiece*)nullptr);
 			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
 			const unsigned items = kBl * w_tiles * r_tiles;
 			const BlmPiece piece{wb, cb, par, (uint32_t)(sh.kw_count / 2), (uint32_t)wc, (uint32_t)r_pad, w_tiles, r_tiles,
 			                     0u, items, sh.row_fast};
-			const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
-			if(e == hipSuccess) e = hipGetLastError();
-			if(e == hipSuccess && sh.rt == 128)
-				hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, piece, (const BlmPiece*)nullptr, items);
-			else if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, piece, (const BlmPiece*)nullptr, items);
-			if(e == hipSuccess) e = hipGetLastError();
+			e = hipGetLastError();
+			if(e == hipSuccess) e = blm_launch_gemm((unsigned)sh.rt, piece, nullptr, items, tab, s);
+			const BlmCombinePiece cp{par, (uint8_t*)out_slots + r0 * slot_bytes + 2 * w0, slot_bytes, (uint32_t)r_pad,
+			                         (uint32_t)(wc / 32), rows, accumulate ? 1u : 0u};
 			if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_combine, combine_grid(rows, wc / 32), dim3(256), 0, s, par, r_pad, wc / 32, rows,
-				                   (unsigned)r0, (uint8_t*)out_slots, slot_bytes, w0, accumulate ? 1 : 0, wtab);
+				hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(rows, wc / 32, 1), dim3(256), 0, s, cp,
+				                   (const BlmCombinePiece*)nullptr, wtab);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e != hipSuccess) return set_error(e);
 		}
 	}
 	return PPH_OK;
+}
+
+// ------------------------------------------------------------ many jobs
+
+// The bilinear jobs of a batch as one sequence of launches: every job's
+// coefficient planes in one launch (one output block per job), then waves of
+// pieces (one word chunk of one job each) whose word planes and parity bits
+// fit the workspace together: per wave one word-plane launch, one GEMM launch
+// per row tile over a cross-job item table, and one recombination launch.
+// Pieces run longest first (input count descending), so that a wave's last
+// items are its shortest.
+struct BlmBatch {
+	std::vector<BlmCoefJob> coef;
+	struct Wave {
+		unsigned words0, words_n;     // slice of the word-piece table
+		size_t words_threads;         // largest piece's thread count
+		unsigned gemm0[2], gemm_n[2]; // slices of the GEMM table (row tile 256, 128)
+		unsigned items[2];
+		unsigned comb0, comb_n;
+		size_t comb_rows, comb_wbk;   // largest piece's
+	};
+	std::vector<Wave> waves;
+	size_t coef_threads = 0;
+	BlmCoefJob* d_coef = nullptr;
+	BlmWordsPiece* d_words = nullptr;
+	BlmPiece* d_gemm = nullptr;
+	BlmCombinePiece* d_comb = nullptr;
+};
+
+namespace {
+
+struct BlmJobShape {
+	size_t kw_count, rt, r_pad, slot_words, cb_bytes;
+};
+
+BlmJobShape blm_job_shape(const BlmBatchJob& j) {
+	BlmJobShape s{};
+	s.kw_count = std::max<size_t>(kBlmMinKw, round_up(j.n_in, kBlmKStep) / 32);
+	s.rt = round_up(j.n_out, 128) < round_up(j.n_out, 256) ? 128 : 256;
+	s.r_pad = round_up(j.n_out, s.rt);
+	s.slot_words = j.slot_bytes / 2;
+	s.cb_bytes = round_up(kBl * s.kw_count * s.r_pad * 4, 256);
+	return s;
+}
+
+// Word chunk of a piece (at most): 65536 words, or PPH_BATCH_PIECE_WORDS
+// rounded up to 512 (a test hook: many pieces from small jobs).
+size_t blm_piece_words() {
+	const size_t v = blm_env("PPH_BATCH_PIECE_WORDS");
+	return v ? round_up(v, kBlmWordTile) : 65536;
+}
+
+size_t blm_piece_bytes(const BlmJobShape& s, size_t wc) {
+	return round_up(kBl * s.kw_count * wc * 4, 256) + round_up(kBl * (wc / 32) * s.r_pad * 4, 256);
+}
+
+}  // namespace
+
+bool blm_batch_eligible(const BlmBatchJob& j, size_t ws_budget) {
+	if(!j.n_in || !j.n_out || j.n_in > 32768 || j.slot_bytes % kSlotAlign) return false;
+	const BlmJobShape s = blm_job_shape(j);
+	const size_t wc = std::min(blm_piece_words(), s.slot_words);
+	// coefficient planes at most a quarter of the workspace, a piece at most a quarter
+	return s.cb_bytes <= ws_budget / 4 && blm_piece_bytes(s, wc) <= ws_budget / 4 &&
+	       blm_planes_fit(s.kw_count, wc, s.r_pad);
+}
+
+size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget) {
+	size_t cb = 0, big = 0, all = 0;
+	const size_t pw = blm_piece_words();
+	for(unsigned k = 0; k < n; k++) {
+		const BlmJobShape s = blm_job_shape(jobs[k]);
+		cb += s.cb_bytes;
+		for(size_t w0 = 0; w0 < s.slot_words; w0 += pw) {
+			const size_t pb = blm_piece_bytes(s, std::min(pw, s.slot_words - w0));
+			big = std::max(big, pb);
+			all += pb;
+		}
+	}
+	if(!n) return 0;
+	// every piece at once when that fits, else the budget (at least the
+	// coefficients and one piece)
+	return std::max(cb + big, std::min(cb + all, ws_budget));
+}
+
+void blm_batch_destroy(BlmBatch* b) {
+	if(!b) return;
+	if(b->d_coef) (void)hipFree(b->d_coef);
+	if(b->d_words) (void)hipFree(b->d_words);
+	if(b->d_gemm) (void)hipFree(b->d_gemm);
+	if(b->d_comb) (void)hipFree(b->d_comb);
+	delete b;
+}
+
+int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, BlmBatch** out) {
+	*out = nullptr;
+	std::unique_ptr<BlmBatch> b(new(std::nothrow) BlmBatch);
+	if(!b) return PPH_E_NOMEM;
+	uint8_t* base = static_cast<uint8_t*>(ws);
+	size_t off = 0;
+	std::vector<BlmJobShape> sh(n);
+	std::vector<uint32_t*> cb(n);
+	for(unsigned k = 0; k < n; k++) {
+		sh[k] = blm_job_shape(jobs[k]);
+		cb[k] = reinterpret_cast<uint32_t*>(base + off);
+		b->coef.push_back(BlmCoefJob{cb[k], jobs[k].in_logs, jobs[k].exps, (uint32_t)sh[k].kw_count,
+		                             (uint32_t)sh[k].r_pad, jobs[k].n_in, jobs[k].n_out});
+		b->coef_threads = std::max(b->coef_threads, sh[k].kw_count * sh[k].r_pad);
+		off += sh[k].cb_bytes;
+	}
+	if(off > ws_bytes) return PPH_E_INVALID;
+	struct Piece {
+		unsigned job;
+		size_t w0, wc;
+	};
+	std::vector<Piece> pieces;
+	const size_t pw = blm_piece_words();
+	for(unsigned k = 0; k < n; k++)
+		for(size_t w0 = 0; w0 < sh[k].slot_words; w0 += pw)
+			pieces.push_back(Piece{k, w0, std::min(pw, sh[k].slot_words - w0)});
+	std::stable_sort(pieces.begin(), pieces.end(),
+	                 [&](const Piece& x, const Piece& y) { return sh[x.job].kw_count > sh[y.job].kw_count; });
+	const size_t area = ws_bytes - off;
+	std::vector<BlmWordsPiece> words;
+	std::vector<BlmPiece> gemm[2];
+	std::vector<BlmCombinePiece> comb;
+	std::vector<BlmPiece> gemm_all;
+	size_t p = 0;
+	while(p < pieces.size()) {
+		BlmBatch::Wave w{};
+		w.words0 = (unsigned)words.size();
+		w.comb0 = (unsigned)comb.size();
+		gemm[0].clear();
+		gemm[1].clear();
+		size_t used = 0;
+		for(; p < pieces.size(); p++) {
+			const Piece& pc = pieces[p];
+			const BlmJobShape& s = sh[pc.job];
+			const size_t wb_bytes = round_up(kBl * s.kw_count * pc.wc * 4, 256);
+			const size_t need = blm_piece_bytes(s, pc.wc);
+			if(used + need > area) {
+				if(!used) return PPH_E_INVALID;  // a piece larger than the area
+				break;
+			}
+			uint32_t* wbp = reinterpret_cast<uint32_t*>(base + off + used);
+			uint32_t* par = reinterpret_cast<uint32_t*>(base + off + used + wb_bytes);
+			used += need;
+			const BlmBatchJob& j = jobs[pc.job];
+			words.push_back(BlmWordsPiece{wbp, (const uint8_t*)j.in_slots + 2 * pc.w0, j.slot_bytes,
+			                              (uint32_t)s.kw_count, (uint32_t)pc.wc, j.n_in, 0u});
+			w.words_threads = std::max(w.words_threads, s.kw_count * pc.wc / 2);
+			const unsigned ti = s.rt == 256 ? 0 : 1;
+			const unsigned w_tiles = (unsigned)(pc.wc / (65536 / s.rt)), r_tiles = (unsigned)(s.r_pad / s.rt);
+			const unsigned items = kBl * w_tiles * r_tiles;
+			gemm[ti].push_back(BlmPiece{wbp, cb[pc.job], par, (uint32_t)(s.kw_count / 2), (uint32_t)pc.wc,
+			                            (uint32_t)s.r_pad, w_tiles, r_tiles, w.items[ti], items, 1u});
+			w.items[ti] += items;
+			comb.push_back(BlmCombinePiece{par, (uint8_t*)j.out_slots + 2 * pc.w0, j.slot_bytes, (uint32_t)s.r_pad,
+			                               (uint32_t)(pc.wc / 32), j.n_out, 0u});
+			w.comb_rows = std::max<size_t>(w.comb_rows, j.n_out);
+			w.comb_wbk = std::max<size_t>(w.comb_wbk, pc.wc / 32);
+		}
+		w.words_n = (unsigned)words.size() - w.words0;
+		w.comb_n = (unsigned)comb.size() - w.comb0;
+		for(unsigned ti = 0; ti < 2; ti++) {
+			w.gemm0[ti] = (unsigned)gemm_all.size();
+			w.gemm_n[ti] = (unsigned)gemm[ti].size();
+			gemm_all.insert(gemm_all.end(), gemm[ti].begin(), gemm[ti].end());
+		}
+		b->waves.push_back(w);
+	}
+	auto upload = [](auto** d, const auto& v) -> hipError_t {
+		if(v.empty()) return hipSuccess;
+		const size_t bytes = v.size() * sizeof(v[0]);
+		hipError_t e = hipMalloc((void**)d, bytes);
+		if(e == hipSuccess) e = hipMemcpy(*d, v.data(), bytes, hipMemcpyHostToDevice);
+		return e;
+	};
+	hipError_t e = upload(&b->d_coef, b->coef);
+	if(e == hipSuccess) e = upload(&b->d_words, words);
+	if(e == hipSuccess) e = upload(&b->d_gemm, gemm_all);
+	if(e == hipSuccess) e = upload(&b->d_comb, comb);
+	if(e != hipSuccess) {
+		blm_batch_destroy(b.release());
+		return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
+	}
+	*out = b.release();
+	return PPH_OK;
+}
+
+int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTables& tab, hipStream_t s) {
+	if(!b || !wtab) return PPH_E_INVALID;
+	if(b->coef.empty()) return PPH_OK;
+	const BlmCoefJob none_c{};
+	const BlmWordsPiece none_w{};
+	const BlmPiece none_g{};
+	const BlmCombinePiece none_m{};
+	hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(b->coef_threads, tab), (unsigned)b->coef.size()), dim3(256), 0, s,
+	                   none_c, b->d_coef, tab.exp);
+	hipError_t e = hipGetLastError();
+	for(const BlmBatch::Wave& w : b->waves) {
+		if(e != hipSuccess) break;
+		hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(w.words_threads, tab), w.words_n), dim3(256), 0, s, none_w,
+		                   b->d_words + w.words0);
+		e = hipGetLastError();
+		for(unsigned ti = 0; ti < 2 && e == hipSuccess; ti++)
+			if(w.gemm_n[ti]) e = blm_launch_gemm(ti ? 128 : 256, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
+		if(e == hipSuccess) {
+			hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(w.comb_rows, w.comb_wbk, w.comb_n), dim3(256), 0, s,
+			                   none_m, b->d_comb + w.comb0, wtab);
+			e = hipGetLastError();
+		}
+	}
+	return e == hipSuccess ? PPH_OK : set_error(e);
 }
 
 }  // namespace pph

@@ -100,6 +100,24 @@ size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, siz
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                 const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap = 0);
+// Many bilinear jobs as one sequence of launches (pph_bl_mfma.hip): waves of
+// word chunks whose planes share the workspace, cross-job GEMM item tables.
+struct BlmBatchJob {
+	const void* in_slots;
+	size_t slot_bytes;
+	unsigned n_in;
+	const uint16_t* in_logs;
+	const uint16_t* exps;
+	void* out_slots;  // overwritten
+	unsigned n_out;
+};
+struct BlmBatch;
+bool blm_batch_eligible(const BlmBatchJob& j, size_t ws_budget);
+size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget);
+int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, BlmBatch** out);
+int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTables& tab, hipStream_t s);
+void blm_batch_destroy(BlmBatch* b);
+const uint16_t* bilinear_wtab();  // recombination table on the current device (after bilinear_prepare)
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_lds_gather_ceiling(const uint16_t* exp_tab, unsigned blocks, unsigned rounds, uint32_t seed,
                                      uint32_t* sink, hipStream_t s);

@@ -116,3 +116,29 @@ def test_autotune_rates_are_sane():
         assert all(v > 1e9 for v in d[k]), k
     # the plain form's rate grows with the recovery count until it is compute bound
     assert d["plain_g16"][4] > d["plain_g16"][0]
+
+
+def test_batch_bilinear_in_waves(refs, monkeypatch):
+    """The bilinear jobs' cross-job plan under a small workspace: 1024-word
+    pieces (53 of them) in waves of a few, each wave one word-plane launch, one
+    GEMM launch over the pieces' items and one recombination launch; eager and
+    graph replays equal the oracle."""
+    monkeypatch.setenv("PPH_BATCH_BILINEAR_WORKSPACE", str(40 << 20))
+    monkeypatch.setenv("PPH_BATCH_PIECE_WORDS", "1024")
+    jobs = make_jobs()
+    batch = B.Batch(jobs, tuning=B.fixed_tuning(plain=0.001, dense=0.002, bilinear=1.0))
+    monkeypatch.delenv("PPH_BATCH_BILINEAR_WORKSPACE")
+    monkeypatch.delenv("PPH_BATCH_PIECE_WORDS")
+    assert all(batch.plan(j)["form"] == "bilinear" for j in range(len(jobs)))
+    for use_graph in (False, True, True):
+        fill_all(jobs)
+        for job in jobs:
+            job.d_out.zero()
+        parpar_amd.check(parpar_amd.lib().pph_synchronize())
+        batch.run(use_graph=use_graph)
+        parpar_amd.check(parpar_amd.lib().pph_synchronize())
+        for j, job in enumerate(jobs):
+            ref, flags = refs[j]
+            assert (job.outputs() == ref).all(), (use_graph, j)
+            assert ((job.valid() == 1) == flags).all(), (use_graph, j)
+    batch.close()
