@@ -527,14 +527,16 @@ def main_batched(args):
     # their fp4 MFMA work, 81 components x 2 ops per mul-add, over the fused
     # region's HIP-event time (which also holds the word-plane, recombination,
     # plain and dense launches).  executed_mfma_ops adds the GEMM's padding
-    # (inputs to 256, at least 512; outputs to its 128- or 256-row tile).
+    # (inputs to 256, at least 512; outputs to its 64-, 128- or 256-row tile).
     bl_ops = bl_exec = 0
     for k, j in enumerate(jobs):
         if forms[k] != "bilinear":
             continue
         words = j.slot // 2
         kw = max(16, -(-j.n_in // 256) * 8)
-        rt = 128 if -(-j.n_out // 128) * 128 < -(-j.n_out // 256) * 256 else 256
+        pads = {256: -(-j.n_out // 256) * 256, 128: -(-j.n_out // 128) * 128, 64: -(-j.n_out // 64) * 64}
+        cost = {256: pads[256], 128: pads[128], 64: 1.42 * pads[64]}  # the library's row-tile rule
+        rt = 256 if cost[256] <= min(cost[128], cost[64]) else (128 if cost[128] <= cost[64] else 64)
         bl_ops += 2 * 81 * j.n_in * j.n_out * words
         bl_exec += 2 * 81 * (32 * kw) * (-(-j.n_out // rt) * rt) * words
     achieved_tf = bl_ops / (fused_ms / 1e3) / 1e12

@@ -74,10 +74,9 @@ unsigned tune_bucket(unsigned r) {
 }
 
 // Share of the bilinear form's padded work that is real: its GEMM pads the
-// inputs to a multiple of 256 and the outputs to its row tile (128, or 256
-// where that pads no more).
+// inputs to a multiple of 256 and the outputs to its row tile (blm_row_tile).
 double bilinear_fill(unsigned n_in, unsigned n_out) {
-	const size_t rt = round_up(n_out, 128) < round_up(n_out, 256) ? 128 : 256;
+	const size_t rt = blm_row_tile(n_out);
 	return (double)n_in / (double)round_up(n_in, 256) * ((double)n_out / (double)round_up(n_out, rt));
 }
 

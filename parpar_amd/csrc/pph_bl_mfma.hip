@@ -124,13 +124,14 @@ __device__ __forceinline__ void blm_transpose32(uint32_t* a) {
 	}
 }
 
-// Word planes of one word chunk of one job: words w0 .. w0 + wc of every input
-// slot (`in` already offset to word w0), wc a multiple of 512.
+// Word planes of one word chunk of one job: words w0 .. w0 + wv of every input
+// slot (`in` already offset to word w0), in plane rows of wc words (wc a
+// multiple of the item's word tile, wv of 512; the rest of a row is zero).
 struct BlmWordsPiece {
 	uint32_t* wb;  // [t][kw_count][wc]
 	const uint8_t* in;
 	uint64_t slot_bytes;
-	uint32_t kw_count, wc, n_in, pad;
+	uint32_t kw_count, wc, wv, n_in;
 };
 
 // Word planes: piece `one`, or piece blockIdx.y of `table`.  Thread: two
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmW
 #pragma unroll
 		for(unsigned j = 0; j < 32; j++) {
 			const size_t i = kw * 32 + 8 * (j % 4) + j / 4;
-			a[j] = i < P.n_in ? reinterpret_cast<const uint32_t*>(P.in + i * P.slot_bytes)[wl / 2] : 0u;
+			a[j] = (i < P.n_in && wl < P.wv) ? reinterpret_cast<const uint32_t*>(P.in + i * P.slot_bytes)[wl / 2] : 0u;
 		}
 		blm_transpose32(a);
 		uint32_t e0[kBl], e1[kBl];
@@ -201,10 +202,12 @@ struct BlmPiece {
 // parity of n is bit 0 of the encoding.
 template <unsigned RT>
 __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* __restrict__ table, unsigned items) {
-	static_assert(RT == 256 || RT == 128, "row tile");
+	static_assert(RT == 256 || RT == 128 || RT == 64, "row tile");
 	constexpr unsigned WT = 65536 / RT;          // words per item
 	constexpr unsigned kA = 2 * WT / 256;        // word masks per thread and step
-	constexpr unsigned kB = 2 * RT / 256;        // coefficient masks per thread and step
+	constexpr unsigned kB = RT == 64 ? 1 : 2 * RT / 256;  // coefficient masks per thread and step (64: half the threads)
+	constexpr unsigned NA = RT == 64 ? 8 : 4;    // 32-word blocks per wave
+	constexpr unsigned NB = RT == 64 ? 2 : 4;    // 32-output blocks per wave
 	constexpr unsigned kBufs = 4, kDepth = 4;
 	__shared__ __align__(16) uint32_t sA[kBufs][2][WT];
 	__shared__ __align__(16) uint32_t sB[kBufs][2][RT];
@@ -214,6 +217,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	const unsigned tid = threadIdx.x, lane = tid & 63;
 	const unsigned wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const unsigned wr = RT == 256 ? wave >> 1 : wave, wcol = RT == 256 ? wave & 1 : 0;
+	const unsigned wa0 = 32 * NA * wr, wb0 = 32 * NB * wcol;  // this wave's first word / output of the item
 	const unsigned half = lane >> 5, row32 = lane & 31;
 
 	// Item descriptors.  A workgroup's items only move forward through the
@@ -246,37 +250,42 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		           P.r_pad};
 	};
 
-	v16f acc[4][4];
+	v16f acc[NA][NB];
 	// LDS layout [group][row]: a half-wave reads 32 consecutive masks.
 	// m[0 .. kA) = word masks (group g = index / (kA / 2), row tid + 256 (index % (kA / 2))),
 	// m[kA .. kA + kB) = coefficient masks (RT = 256: group = index; RT = 128:
-	// group tid / 128, row tid % 128)
+	// group tid / 128, row tid % 128; RT = 64: threads below 128, group tid /
+	// 64, row tid % 64)
 	auto stage = [&](unsigned buf, const uint32_t* m) {
 #pragma unroll
 		for(unsigned q = 0; q < kA; q++) sA[buf][q / (kA / 2)][tid + 256 * (q % (kA / 2))] = m[q];
-		if(RT == 256) {
+		if constexpr(RT == 256) {
 			sB[buf][0][tid] = m[kA];
 			sB[buf][1][tid] = m[kA + 1];
-		} else {
+		} else if constexpr(RT == 128) {
 			sB[buf][tid / 128][tid % 128] = m[kA];
+		} else {
+			if(wave < 2) sB[buf][tid / 64][tid % 64] = m[kA];
 		}
 	};
-	// this lane's masks (group `half`) of its four A rows and four B rows, then
+	// this lane's masks (group `half`) of its NA A rows and NB B rows, then
 	// their e2m1 expansion into MFMA operands
 	auto read_masks = [&](unsigned buf, uint32_t* rm) {
 #pragma unroll
-		for(int i = 0; i < 4; i++) {
-			rm[i] = sA[buf][half][128 * wr + 32 * i + row32];
-			rm[4 + i] = sB[buf][half][128 * wcol + 32 * i + row32];
-		}
+		for(unsigned i = 0; i < NA; i++) rm[i] = sA[buf][half][wa0 + 32 * i + row32];
+#pragma unroll
+		for(unsigned j = 0; j < NB; j++) rm[NA + j] = sB[buf][half][wb0 + 32 * j + row32];
 	};
 	auto expand = [&](const uint32_t* rm, v8i* fa, v8i* fb) {
 #pragma unroll
-		for(int i = 0; i < 4; i++) {
+		for(unsigned i = 0; i < NA; i++) {
 			const uint4 qa = blm_nibbles(rm[i]);
-			const uint4 qb = blm_nibbles_c(rm[4 + i]);
 			fa[i] = v8i{(int)qa.x, (int)qa.y, (int)qa.z, (int)qa.w, 0, 0, 0, 0};
-			fb[i] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
+		}
+#pragma unroll
+		for(unsigned j = 0; j < NB; j++) {
+			const uint4 qb = blm_nibbles_c(rm[NA + j]);
+			fb[j] = v8i{(int)qb.x, (int)qb.y, (int)qb.z, (int)qb.w, 0, 0, 0, 0};
 		}
 	};
 	// Loads are buffer loads: a resource per item and plane (scalar), a byte
@@ -285,26 +294,29 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	// offset in a scalar register, so the address costs no VALU work.  A plane
 	// is < 2^31 bytes (the workspace chunking keeps it so).
 	struct Voff {
-		int v[kA + kB];
+		int v[2 + kB];  // word masks of group 0, 1 (further masks of a group at +1 KiB each), then coefficients
 	};
 	auto voffsets = [&](const Src& f) {
 		Voff o;
-#pragma unroll
-		for(unsigned q = 0; q < kA; q++) o.v[q] = (int)(((q / (kA / 2)) * f.lda + 256 * (q % (kA / 2)) + tid) * 4);
+		o.v[0] = (int)(tid * 4);
+		o.v[1] = (int)((f.lda + tid) * 4);
 		if constexpr(RT == 256) {
-			o.v[kA] = (int)(tid * 4);
-			o.v[kA + 1] = (int)((f.ldb + tid) * 4);
-		} else {
-			o.v[kA] = (int)(((tid / 128) * f.ldb + tid % 128) * 4);
+			o.v[2] = (int)(tid * 4);
+			o.v[3] = (int)((f.ldb + tid) * 4);
+		} else if constexpr(RT == 128) {
+			o.v[2] = (int)(((tid / 128) * f.ldb + tid % 128) * 4);
+		} else {  // threads 128 and up load what threads 0 .. 127 do (in range, not staged)
+			o.v[2] = (int)((((tid / 64) & 1) * f.ldb + tid % 64) * 4);
 		}
 		return o;
 	};
 	auto fetch = [&](uint32_t* m, const Src& f, const Voff& o, unsigned st) {
 		const int sa = (int)((2 * st * f.lda + f.w0) * 4), sb = (int)((2 * st * f.ldb + f.r0) * 4);
 #pragma unroll
-		for(unsigned q = 0; q < kA; q++) m[q] = __builtin_amdgcn_raw_buffer_load_b32(f.a, o.v[q], sa, 0);
+		for(unsigned q = 0; q < kA; q++)
+			m[q] = __builtin_amdgcn_raw_buffer_load_b32(f.a, o.v[q / (kA / 2)] + (int)(1024 * (q % (kA / 2))), sa, 0);
 #pragma unroll
-		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, o.v[kA + q], sb, 0);
+		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, o.v[2 + q], sb, 0);
 	};
 	Src cur = locate(0);
 	Src nxt = locate(n_my > 1 ? 1 : 0);
@@ -317,8 +329,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	// s + 8.  A barrier after every second step: a buffer is written two steps
 	// after its last read and read two steps after its write.
 	uint32_t pf[kDepth][kA + kB];
-	v8i fa[2][4], fb[2][4];
-	uint32_t rm[2][8];
+	v8i fa[2][NA], fb[2][NB];
+	uint32_t rm[2][NA + NB];
 	{
 		uint32_t m0[4][kA + kB];
 #pragma unroll
@@ -349,9 +361,9 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 			read_masks((k + 2) % kBufs, rm[c]);
 			expand(rm[n], fa[n], fb[n]);
 #pragma unroll
-			for(int i = 0; i < 4; i++)
+			for(unsigned i = 0; i < NA; i++)
 #pragma unroll
-				for(int j = 0; j < 4; j++)
+				for(unsigned j = 0; j < NB; j++)
 					acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
 					    fa[c][i], fb[c][j], (decltype(first)::value && k == 0) ? init : acc[i][j], 4, 4, 0, kScale, 0,
 					    kScale);
@@ -362,18 +374,33 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 			// expansion of masks that arrived a step ago), 4 LDS reads (two masks
 			// each) in the first quarter, then the LDS writes (2, or 3 for the
 			// 128-row tile) and the global loads (4, or 5).
+			if constexpr(RT == 64) {
+				// 16 MFMA, 50 VALU (10 masks), 5 LDS reads, 5 LDS writes, 9 loads
 #pragma unroll
-			for(int q = 0; q < 16; q++) {
-				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-				if(q < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
-				if(q & 1)
-					__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // 2 VALU
-				else
-					__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // 3 VALU
-				if(q == 4 || q == 6 || (RT == 128 && q == 8))
-					__builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
-				if(q == 7 || q == 9 || q == 11 || q == 13 || (RT == 128 && q == 15))
-					__builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+				for(int q = 0; q < 16; q++) {
+					__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+					if(q < 5) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+					if(q == 0 || q == 8)
+						__builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+					else
+						__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+					if(q >= 5 && q <= 13 && (q & 1)) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+					if(q >= 6 && q <= 14) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+				}
+			} else {
+#pragma unroll
+				for(int q = 0; q < 16; q++) {
+					__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+					if(q < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 LDS read
+					if(q & 1)
+						__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // 2 VALU
+					else
+						__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // 3 VALU
+					if(q == 4 || q == 6 || (RT == 128 && q == 8))
+						__builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
+					if(q == 7 || q == 9 || q == 11 || q == 13 || (RT == 128 && q == 15))
+						__builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+				}
 			}
 			if(k & 1) __syncthreads();
 		}
@@ -401,11 +428,11 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		// 32, and every read is volatile so that none is scheduled above it
 		// (volatile asm statements keep their order).
 		asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7");
-		const int sp = (int)((((cur.w0 + 128 * wr) / 32) * cur.ldb + cur.r0 + 128 * wcol) * 4);
+		const int sp = (int)((((cur.w0 + wa0) / 32) * cur.ldb + cur.r0 + wb0) * 4);
 #pragma unroll
-		for(int i = 0; i < 4; i++)
+		for(unsigned i = 0; i < NA; i++)
 #pragma unroll
-			for(int j = 0; j < 4; j++) {
+			for(unsigned j = 0; j < NB; j++) {
 				uint32_t u[16];
 #pragma unroll
 				for(int v = 0; v < 16; v++) asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
@@ -429,12 +456,13 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 }
 
 // Recombination of one piece: outputs r0 .. r0 + n_rows, words w0 .. w0 +
-// 32 wbk (`out` already offset to row r0 and word w0).
+// 32 wbk (`out` already offset to row r0 and word w0) from parity planes of
+// plane_wbk 32-word blocks (wbk of them real).
 struct BlmCombinePiece {
-	const uint32_t* par;  // [t][wbk][r_pad]
+	const uint32_t* par;  // [t][plane_wbk][r_pad]
 	uint8_t* out;
 	uint64_t slot_bytes;
-	uint32_t r_pad, wbk, n_rows, accumulate;
+	uint32_t r_pad, wbk, plane_wbk, n_rows, accumulate, pad;
 };
 
 // Recovery words from the parity bits of piece `one`, or of piece blockIdx.z
@@ -450,7 +478,7 @@ __global__ __launch_bounds__(256) void k_blm_combine(BlmCombinePiece one, const 
 	const BlmCombinePiece P = table ? blm_uniform(table[blockIdx.z]) : one;
 	const unsigned r = blockIdx.x * 32 + (threadIdx.x & 31);
 	if(r >= P.n_rows) return;
-	const size_t plane = (size_t)P.wbk * P.r_pad;
+	const size_t plane = (size_t)P.plane_wbk * P.r_pad;
 	for(size_t wb = (size_t)blockIdx.y * 8 + (threadIdx.x >> 5); wb < P.wbk; wb += (size_t)gridDim.y * 8) {
 		uint32_t acc[16];
 #pragma unroll
@@ -513,8 +541,8 @@ BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap)
 	// whole groups of four 64-input steps, at least eight steps
 	s.kw_count = std::max<size_t>(kBlmMinKw, round_up(n_in, kBlmKStep) / 32);
 	s.slot_words = slot_bytes / 2;                // a multiple of 512
-	s.rt = round_up(n_out, 128) < round_up(n_out, 256) ? 128 : 256;
-	if(size_t v = blm_env("PPH_BILINEAR_ROW_TILE")) s.rt = v == 128 ? 128 : 256;
+	s.rt = blm_row_tile(n_out);
+	if(size_t v = blm_env("PPH_BILINEAR_ROW_TILE")) s.rt = v == 64 ? 64 : v == 128 ? 128 : 256;
 	// Items run rows-fastest by default: the 32 items one XCD holds at a time
 	// then share 4 coefficient tiles and 8 word tiles in its L2 (words-fastest:
 	// 2 and 16); 32768 inputs x 64 KiB, 8192 recovery slices ran 2 % faster.
@@ -526,20 +554,21 @@ BlmShape blm_shape(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap)
 	if(cap) rb = std::min(rb, std::max<size_t>(s.rt, cap / 4 / per_row / s.rt * s.rt));
 	s.r_blk = std::min(rb, round_up(n_out, s.rt));
 	const size_t per_word = kBl * s.kw_count * 4 + kBl * s.r_blk / 8;
-	size_t wc = std::max<size_t>(kBlmWordTile, kBlmChunkBudget / per_word / kBlmWordTile * kBlmWordTile);
-	if(size_t c = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(c, kBlmWordTile));
+	const size_t wt = std::max<size_t>(kBlmWordTile, 65536 / s.rt);  // chunks are multiples of the word tile
+	size_t wc = std::max<size_t>(wt, kBlmChunkBudget / per_word / wt * wt);
+	if(size_t c = blm_env("PPH_BILINEAR_CHUNK_WORDS")) wc = std::min(wc, round_up(c, wt));
 	s.cb_bytes = round_up(kBl * s.kw_count * s.r_blk * 4, 256);
 	if(cap) {
 		const size_t used = s.cb_bytes + 512;  // (two more regions rounded to 256)
-		wc = std::min(wc, cap > used ? (cap - used) / per_word / kBlmWordTile * kBlmWordTile : 0);
-		if(wc < kBlmWordTile) {
+		wc = std::min(wc, cap > used ? (cap - used) / per_word / wt * wt : 0);
+		if(wc < wt) {
 			s.total = SIZE_MAX;
 			return s;
 		}
 	}
-	wc = std::min(wc, round_up(s.slot_words, kBlmWordTile));
+	wc = std::min(wc, round_up(s.slot_words, wt));
 	const size_t chunks = (s.slot_words + wc - 1) / wc;
-	s.wc = round_up((s.slot_words + chunks - 1) / chunks, kBlmWordTile);
+	s.wc = round_up((s.slot_words + chunks - 1) / chunks, wt);
 	s.wb_bytes = round_up(kBl * s.kw_count * s.wc * 4, 256);
 	s.p_bytes = round_up(kBl * s.r_blk * (s.wc / 32) * 4, 256);
 	s.total = s.cb_bytes + s.wb_bytes + s.p_bytes;
@@ -569,7 +598,9 @@ hipError_t blm_launch_gemm(unsigned rt, const BlmPiece& one, const BlmPiece* tab
                            const DeviceTables& tab, hipStream_t s) {
 	if(!items) return hipSuccess;
 	const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
-	if(rt == 128)
+	if(rt == 64)
+		hipLaunchKernelGGL(k_blm_gemm<64>, dim3(grid), dim3(256), 0, s, one, table, items);
+	else if(rt == 128)
 		hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, one, table, items);
 	else
 		hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, one, table, items);
@@ -601,9 +632,10 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 		hipError_t e = hipGetLastError();
 		if(e != hipSuccess) return set_error(e);
 		for(size_t w0 = 0; w0 < sh.slot_words; w0 += sh.wc) {
-			const size_t wc = std::min(sh.wc, sh.slot_words - w0);  // a multiple of 512
+			const size_t wv = std::min(sh.wc, sh.slot_words - w0);          // a multiple of 512
+			const size_t wc = round_up(wv, std::max<size_t>(kBlmWordTile, 65536 / sh.rt));  // <= sh.wc
 			const BlmWordsPiece wp{wb, (const uint8_t*)in_slots + 2 * w0, slot_bytes, (uint32_t)sh.kw_count,
-			                       (uint32_t)wc, n_in, 0u};
+			                       (uint32_t)wc, (uint32_t)wv, n_in};
 			hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(sh.kw_count * wc / 2, tab)), dim3(256), 0, s, wp,
 			                   (const BlmWordsPiece*)nullptr);
 			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
@@ -612,10 +644,13 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 			                     0u, items, sh.row_fast};
 			e = hipGetLastError();
 			if(e == hipSuccess) e = blm_launch_gemm((unsigned)sh.rt, piece, nullptr, items, tab, s);
-			const BlmCombinePiece cp{par, (uint8_t*)out_slots + r0 * slot_bytes + 2 * w0, slot_bytes, (uint32_t)r_pad,
-			                         (uint32_t)(wc / 32), rows, accumulate ? 1u : 0u};
+			const BlmCombinePiece cp{par,           (uint8_t*)out_slots + r0 * slot_bytes + 2 * w0,
+			                         slot_bytes,    (uint32_t)r_pad,
+			                         (uint32_t)(wv / 32), (uint32_t)(wc / 32),
+			                         rows,          accumulate ? 1u : 0u,
+			                         0u};
 			if(e == hipSuccess)
-				hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(rows, wc / 32, 1), dim3(256), 0, s, cp,
+				hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(rows, wv / 32, 1), dim3(256), 0, s, cp,
 				                   (const BlmCombinePiece*)nullptr, wtab);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e != hipSuccess) return set_error(e);
@@ -638,8 +673,8 @@ struct BlmBatch {
 	struct Wave {
 		unsigned words0, words_n;     // slice of the word-piece table
 		size_t words_threads;         // largest piece's thread count
-		unsigned gemm0[2], gemm_n[2]; // slices of the GEMM table (row tile 256, 128)
-		unsigned items[2];
+		unsigned gemm0[3], gemm_n[3]; // slices of the GEMM table (row tile 256, 128, 64)
+		unsigned items[3];
 		unsigned comb0, comb_n;
 		size_t comb_rows, comb_wbk;   // largest piece's
 	};
@@ -654,13 +689,14 @@ struct BlmBatch {
 namespace {
 
 struct BlmJobShape {
-	size_t kw_count, rt, r_pad, slot_words, cb_bytes;
+	size_t kw_count, rt, wt, r_pad, slot_words, cb_bytes;  // wt: word tile, plane rows are multiples
 };
 
 BlmJobShape blm_job_shape(const BlmBatchJob& j) {
 	BlmJobShape s{};
 	s.kw_count = std::max<size_t>(kBlmMinKw, round_up(j.n_in, kBlmKStep) / 32);
-	s.rt = round_up(j.n_out, 128) < round_up(j.n_out, 256) ? 128 : 256;
+	s.rt = blm_row_tile(j.n_out);
+	s.wt = std::max<size_t>(kBlmWordTile, 65536 / s.rt);
 	s.r_pad = round_up(j.n_out, s.rt);
 	s.slot_words = j.slot_bytes / 2;
 	s.cb_bytes = round_up(kBl * s.kw_count * s.r_pad * 4, 256);
@@ -674,7 +710,9 @@ size_t blm_piece_words() {
 	return v ? round_up(v, kBlmWordTile) : 65536;
 }
 
-size_t blm_piece_bytes(const BlmJobShape& s, size_t wc) {
+// planes of a piece of wv words (its rows padded to the word tile)
+size_t blm_piece_bytes(const BlmJobShape& s, size_t wv) {
+	const size_t wc = round_up(wv, s.wt);
 	return round_up(kBl * s.kw_count * wc * 4, 256) + round_up(kBl * (wc / 32) * s.r_pad * 4, 256);
 }
 
@@ -686,7 +724,7 @@ bool blm_batch_eligible(const BlmBatchJob& j, size_t ws_budget) {
 	const size_t wc = std::min(blm_piece_words(), s.slot_words);
 	// coefficient planes at most a quarter of the workspace, a piece at most a quarter
 	return s.cb_bytes <= ws_budget / 4 && blm_piece_bytes(s, wc) <= ws_budget / 4 &&
-	       blm_planes_fit(s.kw_count, wc, s.r_pad);
+	       blm_planes_fit(s.kw_count, round_up(wc, s.wt), s.r_pad);
 }
 
 size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget) {
@@ -746,7 +784,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	                 [&](const Piece& x, const Piece& y) { return sh[x.job].kw_count > sh[y.job].kw_count; });
 	const size_t area = ws_bytes - off;
 	std::vector<BlmWordsPiece> words;
-	std::vector<BlmPiece> gemm[2];
+	std::vector<BlmPiece> gemm[3];
 	std::vector<BlmCombinePiece> comb;
 	std::vector<BlmPiece> gemm_all;
 	size_t p = 0;
@@ -754,13 +792,13 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 		BlmBatch::Wave w{};
 		w.words0 = (unsigned)words.size();
 		w.comb0 = (unsigned)comb.size();
-		gemm[0].clear();
-		gemm[1].clear();
+		for(auto& g : gemm) g.clear();
 		size_t used = 0;
 		for(; p < pieces.size(); p++) {
 			const Piece& pc = pieces[p];
 			const BlmJobShape& s = sh[pc.job];
-			const size_t wb_bytes = round_up(kBl * s.kw_count * pc.wc * 4, 256);
+			const size_t wc = round_up(pc.wc, s.wt);  // plane row: pc.wc words, padded to the word tile
+			const size_t wb_bytes = round_up(kBl * s.kw_count * wc * 4, 256);
 			const size_t need = blm_piece_bytes(s, pc.wc);
 			if(used + need > area) {
 				if(!used) return PPH_E_INVALID;  // a piece larger than the area
@@ -771,22 +809,22 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			used += need;
 			const BlmBatchJob& j = jobs[pc.job];
 			words.push_back(BlmWordsPiece{wbp, (const uint8_t*)j.in_slots + 2 * pc.w0, j.slot_bytes,
-			                              (uint32_t)s.kw_count, (uint32_t)pc.wc, j.n_in, 0u});
-			w.words_threads = std::max(w.words_threads, s.kw_count * pc.wc / 2);
-			const unsigned ti = s.rt == 256 ? 0 : 1;
-			const unsigned w_tiles = (unsigned)(pc.wc / (65536 / s.rt)), r_tiles = (unsigned)(s.r_pad / s.rt);
+			                              (uint32_t)s.kw_count, (uint32_t)wc, (uint32_t)pc.wc, j.n_in});
+			w.words_threads = std::max(w.words_threads, s.kw_count * wc / 2);
+			const unsigned ti = s.rt == 256 ? 0 : s.rt == 128 ? 1 : 2;
+			const unsigned w_tiles = (unsigned)(wc / (65536 / s.rt)), r_tiles = (unsigned)(s.r_pad / s.rt);
 			const unsigned items = kBl * w_tiles * r_tiles;
-			gemm[ti].push_back(BlmPiece{wbp, cb[pc.job], par, (uint32_t)(s.kw_count / 2), (uint32_t)pc.wc,
+			gemm[ti].push_back(BlmPiece{wbp, cb[pc.job], par, (uint32_t)(s.kw_count / 2), (uint32_t)wc,
 			                            (uint32_t)s.r_pad, w_tiles, r_tiles, w.items[ti], items, 1u});
 			w.items[ti] += items;
 			comb.push_back(BlmCombinePiece{par, (uint8_t*)j.out_slots + 2 * pc.w0, j.slot_bytes, (uint32_t)s.r_pad,
-			                               (uint32_t)(pc.wc / 32), j.n_out, 0u});
+			                               (uint32_t)(pc.wc / 32), (uint32_t)(wc / 32), j.n_out, 0u, 0u});
 			w.comb_rows = std::max<size_t>(w.comb_rows, j.n_out);
 			w.comb_wbk = std::max<size_t>(w.comb_wbk, pc.wc / 32);
 		}
 		w.words_n = (unsigned)words.size() - w.words0;
 		w.comb_n = (unsigned)comb.size() - w.comb0;
-		for(unsigned ti = 0; ti < 2; ti++) {
+		for(unsigned ti = 0; ti < 3; ti++) {
 			w.gemm0[ti] = (unsigned)gemm_all.size();
 			w.gemm_n[ti] = (unsigned)gemm[ti].size();
 			gemm_all.insert(gemm_all.end(), gemm[ti].begin(), gemm[ti].end());
@@ -827,8 +865,9 @@ int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTable
 		hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(w.words_threads, tab), w.words_n), dim3(256), 0, s, none_w,
 		                   b->d_words + w.words0);
 		e = hipGetLastError();
-		for(unsigned ti = 0; ti < 2 && e == hipSuccess; ti++)
-			if(w.gemm_n[ti]) e = blm_launch_gemm(ti ? 128 : 256, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
+		for(unsigned ti = 0; ti < 3 && e == hipSuccess; ti++)
+			if(w.gemm_n[ti])
+				e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
 		if(e == hipSuccess) {
 			hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(w.comb_rows, w.comb_wbk, w.comb_n), dim3(256), 0, s,
 			                   none_m, b->d_comb + w.comb0, wtab);

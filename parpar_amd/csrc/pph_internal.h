@@ -100,6 +100,17 @@ size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, siz
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                 const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap = 0);
+// Row tile of the bilinear GEMM for n_out outputs: the one that pads the
+// outputs least, a 64-row tile's rows weighed 1.42 (measured: its waves expand
+// ten masks per step where the others expand eight, and issue twice the
+// loads), ties to the larger tile.  So 64 only up to 64 outputs.
+inline size_t blm_row_tile(size_t n_out) {
+	const auto ru = [](size_t v, size_t a) { return (v + a - 1) / a * a; };
+	const double c256 = (double)ru(n_out, 256), c128 = (double)ru(n_out, 128), c64 = 1.42 * (double)ru(n_out, 64);
+	if(c256 <= c128 && c256 <= c64) return 256;
+	return c128 <= c64 ? 128 : 64;
+}
+
 // Many bilinear jobs as one sequence of launches (pph_bl_mfma.hip): waves of
 // word chunks whose planes share the workspace, cross-job GEMM item tables.
 struct BlmBatchJob {
