@@ -265,9 +265,9 @@ int pph_bilinear_release(void);
  * hipMalloc, hipFree or other allocation: the GEMM's planes and parity bits
  * live in a static device array of PPH_NOALLOC_SCRATCH_BYTES per device (the
  * work is chunked to fit it; every n_in <= 32768 fits, a shape that would not
- * returns PPH_E_INVALID), and the exp and recombination tables in static
- * device arrays.  The first call on a device finds the arrays' addresses and
- * fills the tables (on `stream`, waited for).  Every call on a device uses
+ * returns PPH_E_INVALID), and the exp table in a static device array.  The
+ * first call on a device finds the arrays' addresses and fills the table (on
+ * `stream`, waited for).  Every call on a device uses
  * the same arrays: calls that overlap on one GPU race, so serialise them
  * (one stream, or events between streams). */
 #define PPH_NOALLOC_SCRATCH_BYTES (256ull << 20)

@@ -286,7 +286,7 @@ struct pph_batch {
 // others (too large for its workspace share) one by one.
 static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s) {
 	if(b->bl_plan) {
-		const int st = blm_batch_enqueue(b->bl_plan, bilinear_wtab(), *b->tab, s);
+		const int st = blm_batch_enqueue(b->bl_plan, *b->tab, s);
 		if(st) return hipErrorLaunchFailure;
 	}
 	for(size_t j = 0; j < b->jobs.size(); j++) {
@@ -420,7 +420,7 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 			best = td;
 			b->form[j] = PPH_FORM_DENSE;
 		}
-		if(tb < best && bilinear_prepare(jb.slot_bytes, jb.n_in, jb.n_out) == PPH_OK) {
+		if(tb < best && bilinear_supported(jb.slot_bytes, jb.n_in, jb.n_out) == PPH_OK) {
 			best = tb;
 			b->form[j] = PPH_FORM_BILINEAR;
 		}

@@ -51,7 +51,6 @@ struct BlDevice {
 	std::mutex mu;
 	uint8_t* buf = nullptr;  // workspace of pph_muladd_bilinear: coefficient planes | word planes | parity bits
 	size_t buf_bytes = 0;
-	uint16_t* d_wtab = nullptr;
 	hipEvent_t tail = nullptr;  // end of the last call on the shared workspace
 	bool tail_valid = false;
 	std::mutex call_mu;         // held across one call's wait, enqueue and record
@@ -110,46 +109,20 @@ size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out
 	return blm_workspace_bytes(slot_bytes, n_in, n_out);
 }
 
-// The recombination table on the current device.  Host work only: run it
-// before a stream capture that encloses the enqueue.
-int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out) {
+// Whether the bilinear form takes this shape (a host check: no device work).
+int bilinear_supported(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 	if(!n_in || !n_out || n_in > 32768 || slot_bytes % kSlotAlign) return PPH_E_INVALID;
-	int dev = 0;
-	hipError_t e = hipGetDevice(&dev);
-	if(e != hipSuccess) return set_error(e);
-	BlDevice* d = bl_device(dev);
-	std::lock_guard<std::mutex> lk(d->mu);
-	if(!d->d_wtab) {
-		uint16_t* w = nullptr;
-		e = hipMalloc((void**)&w, kBl * sizeof(uint16_t));
-		if(e == hipSuccess) e = hipMemcpy(w, bl_recombine_table(), kBl * sizeof(uint16_t), hipMemcpyHostToDevice);
-		if(e != hipSuccess) {
-			if(w) (void)hipFree(w);
-			return set_error(e);
-		}
-		d->d_wtab = w;
-	}
 	return PPH_OK;
 }
 
-const uint16_t* bilinear_wtab() {
-	int dev = 0;
-	if(hipGetDevice(&dev) != hipSuccess) return nullptr;
-	BlDevice* d = bl_device(dev);
-	std::lock_guard<std::mutex> lk(d->mu);
-	return d->d_wtab;
-}
-
 // Enqueue one bilinear mul-add on s with a caller-owned workspace of at least
-// bilinear_workspace_bytes.  bilinear_prepare must have succeeded on this device.
+// bilinear_workspace_bytes.
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
                      const DeviceTables& tab, hipStream_t s) {
 	if(!n_out) return PPH_OK;
-	const uint16_t* wtab = bilinear_wtab();
-	if(!wtab) return PPH_E_STATE;
-	return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate, ws, ws_bytes, wtab,
-	                   tab, s);
+	return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate, ws, ws_bytes, tab,
+	                   s);
 }
 
 }  // namespace pph
@@ -183,11 +156,11 @@ int reserve_locked(BlDevice* d, size_t need) {
 
 }  // namespace
 
-// The recombination table and a shared workspace large enough for the shape on
-// the current device: after PPH_OK, pph_muladd_bilinear of that shape
-// allocates nothing.  PPH_E_NOMEM when the workspace does not fit.
+// A shared workspace large enough for the shape on the current device: after
+// PPH_OK, pph_muladd_bilinear of that shape allocates nothing.  PPH_E_NOMEM
+// when the workspace does not fit.
 int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out) {
-	int rc = bilinear_prepare(slot_bytes, n_in, n_out);
+	int rc = bilinear_supported(slot_bytes, n_in, n_out);
 	if(rc) return rc;
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
@@ -204,16 +177,15 @@ int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 //
 // pph_muladd_bilinear_noalloc keeps every buffer its kernels use beyond the
 // caller's inputs and outputs in static device memory of this file: the
-// GEMM's planes and parity bits in `scratch` (chunked to fit), the exp table
-// in `scratch_exp` and the recombination table in `scratch_wtab` (both filled
-// by the first call on each device).  Each GPU holds its own copy of these
+// GEMM's planes and parity bits in `scratch` (chunked to fit) and the exp
+// table in `scratch_exp` (filled by the first call on each device).  Each GPU
+// holds its own copy of these
 // arrays for the life of the process; in host code their names are
 // stand-ins, so their device addresses come from hipGetSymbolAddress, once
 // per device, and only those addresses are used there.
 
 static __device__ __align__(256) uint8_t scratch[PPH_NOALLOC_SCRATCH_BYTES];
 static __device__ __align__(256) uint16_t scratch_exp[65536];
-static __device__ __align__(256) uint16_t scratch_wtab[pph::kBl];
 
 namespace pph {
 namespace {
@@ -222,21 +194,15 @@ struct StaticScratch {
 	bool ready = false;
 	uint8_t* ws = nullptr;
 	uint16_t* exp = nullptr;
-	const uint16_t* wtab = nullptr;
 	int cus = 0;
 };
 std::mutex g_scratch_mu;
 StaticScratch g_scratch[64];
 
-// The static tables: scratch_exp[k] = 2^k in GF(2^16) (k < 65535),
-// scratch_exp[65535] = 0 (the tables' zero entry); scratch_wtab = the
-// recombination words of the bilinear formula.
+// The static exp table: scratch_exp[k] = 2^k in GF(2^16) (k < 65535),
+// scratch_exp[65535] = 0 (the tables' zero entry).
 __global__ __launch_bounds__(256) void k_fill_scratch_tables() {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-	if(k < kBl) {
-		constexpr BlTable kW = bl_table();
-		scratch_wtab[k] = kW.w[k];
-	}
 	if(k > 65535) return;
 	if(k == 65535) {
 		scratch_exp[k] = 0;
@@ -267,10 +233,8 @@ int scratch_setup(int dev, hipStream_t s, StaticScratch** out) {
 	if(!st.ready) {
 		void* ws = nullptr;
 		void* ex = nullptr;
-		void* wt = nullptr;
 		hipError_t e = hipGetSymbolAddress(&ws, HIP_SYMBOL(scratch));
 		if(e == hipSuccess) e = hipGetSymbolAddress(&ex, HIP_SYMBOL(scratch_exp));
-		if(e == hipSuccess) e = hipGetSymbolAddress(&wt, HIP_SYMBOL(scratch_wtab));
 		int cus = 0;
 		if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 		if(e != hipSuccess) return set_error(e);
@@ -280,7 +244,6 @@ int scratch_setup(int dev, hipStream_t s, StaticScratch** out) {
 		if(e != hipSuccess) return set_error(e);
 		st.ws = (uint8_t*)ws;
 		st.exp = (uint16_t*)ex;
-		st.wtab = (const uint16_t*)wt;
 		st.cus = cus;
 		st.ready = true;
 	}
@@ -310,7 +273,7 @@ extern "C" int pph_muladd_bilinear_noalloc(const void* in_slots, size_t slot_byt
 	t.exp = st->exp;
 	t.cus = st->cus;
 	return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, st->ws,
-	                   PPH_NOALLOC_SCRATCH_BYTES, st->wtab, t, (hipStream_t)stream, PPH_NOALLOC_SCRATCH_BYTES);
+	                   PPH_NOALLOC_SCRATCH_BYTES, t, (hipStream_t)stream, PPH_NOALLOC_SCRATCH_BYTES);
 }
 
 extern "C" int pph_debug_bilinear_workspace_limit(size_t bytes) {
@@ -330,7 +293,7 @@ extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsi
 	DeviceTables* tab = nullptr;
 	int rc = device_tables(&tab);
 	if(rc) return rc;
-	rc = bilinear_prepare(slot_bytes, n_in, n_out);
+	rc = bilinear_supported(slot_bytes, n_in, n_out);
 	if(rc) return rc;
 	BlDevice* d = bl_device(dev);
 	std::lock_guard<std::mutex> call(d->call_mu);

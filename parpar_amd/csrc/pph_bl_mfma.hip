@@ -134,31 +134,51 @@ struct BlmWordsPiece {
 	uint32_t kw_count, wc, wv, n_in;
 };
 
-// Word planes: piece `one`, or piece blockIdx.y of `table`.  Thread: two
-// adjacent words, one 32-input group; words vary fastest so each of the 32
-// slot reads (4 bytes) is coalesced.  Row j of the 32 x 32 bit matrix is input
-// 8 (j % 4) + j / 4 (the packed position order), so after the transpose row
-// b < 16 is bit slice b of the first word and row 16 + b that of the second.
+// Word planes: piece `one`, or piece blockIdx.y of `table`.  A workgroup
+// covers 512 words of one 32-input group (chunks are multiples of 512 words,
+// so the group, the unit's first word and whether it holds data are uniform:
+// loads and stores take a scalar base and a 32-bit lane offset).  Thread: two
+// adjacent words; each of the 32 slot reads (4 bytes) is coalesced.  Row j of
+// the 32 x 32 bit matrix is input 8 (j % 4) + j / 4 (the packed position
+// order), so after the transpose row b < 16 is bit slice b of the first word
+// and row 16 + b that of the second.
 __global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmWordsPiece* __restrict__ table) {
 	const BlmWordsPiece P = table ? blm_uniform(table[blockIdx.y]) : one;
-	const size_t pairs = P.wc / 2;
-	const size_t total = (size_t)P.kw_count * pairs;
-	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const size_t kw = g / pairs;
-		const size_t wl = 2 * (g - kw * pairs);
+	const unsigned per_kw = P.wc / 512;
+	const size_t units = (size_t)P.kw_count * per_kw;
+	const size_t plane = (size_t)P.kw_count * P.wc;
+	const unsigned tid = threadIdx.x;
+	for(size_t u = blockIdx.x; u < units; u += gridDim.x) {
+		const unsigned kw = (unsigned)(u / per_kw);
+		const unsigned w0 = 512 * (unsigned)(u - (size_t)kw * per_kw);
 		uint32_t a[32];
+		if(w0 < P.wv) {
+			const uint8_t* in = P.in + 2 * (size_t)w0 + (size_t)kw * 32 * P.slot_bytes;
+			const unsigned n = P.n_in > kw * 32 ? P.n_in - kw * 32 : 0;  // inputs of this group
 #pragma unroll
-		for(unsigned j = 0; j < 32; j++) {
-			const size_t i = kw * 32 + 8 * (j % 4) + j / 4;
-			a[j] = (i < P.n_in && wl < P.wv) ? reinterpret_cast<const uint32_t*>(P.in + i * P.slot_bytes)[wl / 2] : 0u;
+			for(unsigned k = 0; k < 32; k++) {
+				// past the last input: an empty resource, whose loads return 0
+				a[4 * (k % 8) + k / 8] = __builtin_amdgcn_raw_buffer_load_b32(
+				    __builtin_amdgcn_make_buffer_rsrc((void*)in, 0, k < n ? 2048 : 0, 0x00020000), (int)(4 * tid), 0, 0);
+				in += P.slot_bytes;
+			}
+		} else {
+#pragma unroll
+			for(unsigned j = 0; j < 32; j++) a[j] = 0;
 		}
 		blm_transpose32(a);
 		uint32_t e0[kBl], e1[kBl];
 		bl_expand_slices(a, e0);
 		bl_expand_slices(a + 16, e1);
+		uint32_t* dst = P.wb + (size_t)kw * P.wc + w0;  // uniform
 #pragma unroll
-		for(unsigned t = 0; t < kBl; t++)
-			*reinterpret_cast<uint2*>(P.wb + ((size_t)t * P.kw_count + kw) * P.wc + wl) = make_uint2(e0[t], e1[t]);
+		for(unsigned t = 0; t < kBl; t++) {
+			typedef unsigned v2u __attribute__((ext_vector_type(2)));
+			__builtin_amdgcn_raw_buffer_store_b64(v2u{e0[t], e1[t]},
+			                                      __builtin_amdgcn_make_buffer_rsrc(dst, 0, 2048, 0x00020000),
+			                                      (int)(8 * tid), 0, 0);
+			dst += plane;
+		}
 	}
 }
 
@@ -225,7 +245,9 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	BlmPiece P = table ? blm_uniform(table[0]) : one;
 	unsigned pc = 0;
 	struct Src {
-		__amdgpu_buffer_rsrc_t a, b, p;
+		__amdgpu_buffer_rsrc_t a, b;
+		uint32_t* pbase;
+		uint32_t pbytes;
 		unsigned w0, r0, steps, lda, ldb;
 	};
 	constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit access, gfx9 family
@@ -242,7 +264,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		const size_t pp = (size_t)(P.wc_pad / 32) * P.r_pad;
 		return Src{__builtin_amdgcn_make_buffer_rsrc((void*)(P.wb + t * pa), 0, (int)(pa * 4), kRsrcWord3),
 		           __builtin_amdgcn_make_buffer_rsrc((void*)(P.cb + t * pb), 0, (int)(pb * 4), kRsrcWord3),
-		           __builtin_amdgcn_make_buffer_rsrc((void*)(P.par + t * pp), 0, (int)(pp * 4), kRsrcWord3),
+		           P.par + t * pp,
+		           (uint32_t)(pp * 4),
 		           wt * WT,
 		           rt * RT,
 		           P.steps,
@@ -346,6 +369,33 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	read_masks(1, rm[1]);
 	__syncthreads();  // buffers 0 and 1 are written again in the first two steps
 
+	// The parity words of an item are stored during the first step of the next
+	// item, after that step's staging: vector memory operations complete in
+	// order, so a staging wait issued behind the stores would wait for them too.
+	// Before the first item the resource is empty, so its stores are dropped.
+	// The destination is made wave-uniform where it is used (a loop-carried
+	// value the compiler would otherwise treat as per-lane, looping over lanes).
+	uint32_t po[NA * NB];
+#pragma unroll
+	for(unsigned q = 0; q < NA * NB; q++) po[q] = 0;
+	struct Dst {
+		uint32_t* base;  // parity plane of the item's component
+		uint32_t bytes;  // its size (0: no item)
+		int sp;          // byte offset of this wave's first block
+		uint32_t ldb;
+	};
+	Dst pd{P.par, 0u, 0, 0u};
+	auto store_prev = [&]() {
+		const Dst d = blm_uniform(pd);
+		const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)d.base, 0, (int)d.bytes, kRsrcWord3);
+#pragma unroll
+		for(unsigned i = 0; i < NA; i++)
+#pragma unroll
+			for(unsigned j = 0; j < NB; j++)
+				__builtin_amdgcn_raw_buffer_store_b32(po[i * NB + j], rs, (int)(row32 * 4),
+				                                      d.sp + (int)((i * d.ldb + 32 * j) * 4), 0);
+	};
+
 	constexpr int kScale = 116;  // e8m0 2^-11 on both operands
 	const v16f init = {2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f,
 	                   2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f};
@@ -368,6 +418,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 					    fa[c][i], fb[c][j], (decltype(first)::value && k == 0) ? init : acc[i][j], 4, 4, 0, kScale, 0,
 					    kScale);
 			stage((k + 4) % kBufs, pf[k]);
+			if constexpr(decltype(first)::value)
+				if(k == 0) store_prev();
 			fetch(pf[k], f, o, st0 + k);
 			// Every MFMA followed by a few instructions of the other kinds, spread
 			// evenly over the step, in the counts the step has: 40 VALU (the
@@ -386,6 +438,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 						__builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
 					if(q >= 5 && q <= 13 && (q & 1)) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
 					if(q >= 6 && q <= 14) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+					if(decltype(first)::value && k == 0) __builtin_amdgcn_sched_group_barrier(0x040, 1, 0);
 				}
 			} else {
 #pragma unroll
@@ -400,6 +453,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 						__builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // 1 LDS write
 					if(q == 7 || q == 9 || q == 11 || q == 13 || (RT == 128 && q == 15))
 						__builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 global load
+					if(decltype(first)::value && k == 0)
+						__builtin_amdgcn_sched_group_barrier(0x040, 1, 0);  // 1 store (previous item)
 				}
 			}
 			if(k & 1) __syncthreads();
@@ -408,10 +463,16 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	for(unsigned cj = 0; cj < n_my; cj++) {
 		// the first pass fetches steps 8 .. 11: of the next item when this one
 		// has eight steps
-		if(cur.steps > 8)
-			pass(0u, std::true_type{}, cur, vc, 8u);
-		else
-			pass(0u, std::true_type{}, nxt, vn, 0u);
+		{
+			const bool more = cur.steps > 8;
+			Src f0 = nxt;
+			Voff o0 = vn;
+			if(more) {
+				f0 = cur;
+				o0 = vc;
+			}
+			pass(0u, std::true_type{}, f0, o0, more ? 8u : 0u);
+		}
 		unsigned s0 = 4;
 		for(; s0 + 4 + kDepth < cur.steps; s0 += 4) pass(s0, std::false_type{}, cur, vc, s0 + 4 + kDepth);
 		for(; s0 < cur.steps; s0 += 4) pass(s0, std::false_type{}, nxt, vn, s0 + 4 + kDepth - cur.steps);
@@ -444,15 +505,15 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 					m |= (w & 0x01010101u) << (4 * half + c);
 				}
 				const auto sw = __builtin_amdgcn_permlane32_swap(m, m, false, false);
-				m = sw[0] | sw[1];
-				__builtin_amdgcn_raw_buffer_store_b32(m, cur.p, (int)(row32 * 4), sp + (int)((i * cur.ldb + 32 * j) * 4),
-				                                      0);
+				po[i * NB + j] = sw[0] | sw[1];
 			}
+		pd = Dst{cur.pbase, cur.pbytes, sp, cur.ldb};
 		cur = nxt;
 		vc = vn;
 		nxt = locate(cj + 2 < n_my ? cj + 2 : n_my - 1);
 		vn = voffsets(nxt);
 	}
+	store_prev();
 }
 
 // Recombination of one piece: outputs r0 .. r0 + n_rows, words w0 .. w0 +
@@ -467,30 +528,30 @@ struct BlmCombinePiece {
 
 // Recovery words from the parity bits of piece `one`, or of piece blockIdx.z
 // of `table`: a block of 32 output rows x 8 32-word blocks, rows varying
-// fastest within a wave (coalesced parity reads); the 16 result bit-slices are
-// XORs of the parity masks selected by the recombination table.  Words 2i and
+// fastest within a wave (coalesced parity reads); the 16 result bit-slices come
+// from the 81 parity masks by the Karatsuba recombination and the reduction
+// (bl_recombine, bl_reduce_slices: XORs fixed at compile time).  Words 2i and
 // 2i + 1 go to one dword: a 32 x 32 bit transpose of the slices' even bits
 // (rows 0 .. 15) and odd bits shifted down (rows 16 .. 31) leaves that dword
 // in row 2i.  Each thread stores its 64 contiguous bytes as four 16-byte
 // stores (slots are 1 KiB multiples, chunks 512-word multiples).
-__global__ __launch_bounds__(256) void k_blm_combine(BlmCombinePiece one, const BlmCombinePiece* __restrict__ table,
-                                                     const uint16_t* __restrict__ wtab) {
+__global__ __launch_bounds__(256) void k_blm_combine(BlmCombinePiece one, const BlmCombinePiece* __restrict__ table) {
 	const BlmCombinePiece P = table ? blm_uniform(table[blockIdx.z]) : one;
 	const unsigned r = blockIdx.x * 32 + (threadIdx.x & 31);
 	if(r >= P.n_rows) return;
 	const size_t plane = (size_t)P.plane_wbk * P.r_pad;
 	for(size_t wb = (size_t)blockIdx.y * 8 + (threadIdx.x >> 5); wb < P.wbk; wb += (size_t)gridDim.y * 8) {
-		uint32_t acc[16];
+		const __attribute__((address_space(1))) uint32_t* src =
+		    (const __attribute__((address_space(1))) uint32_t*)(P.par + wb * P.r_pad + r);
+		uint32_t pm[kBl];
 #pragma unroll
-		for(int j = 0; j < 16; j++) acc[j] = 0;
-		const uint32_t* src = P.par + wb * P.r_pad + r;
-#pragma unroll 9
 		for(unsigned t = 0; t < kBl; t++) {
-			const uint32_t m = src[t * plane];
-			const uint32_t w = wtab[t];
-#pragma unroll
-			for(int j = 0; j < 16; j++) acc[j] ^= m & (0u - ((w >> j) & 1u));
+			pm[t] = *src;
+			src += plane;
 		}
+		uint32_t acc[31];
+		bl_recombine<16>(pm, acc);
+		bl_reduce_slices(acc);
 		uint32_t a[32];
 #pragma unroll
 		for(int j = 0; j < 16; j++) {
@@ -615,10 +676,10 @@ dim3 blm_combine_grid(size_t rows, size_t wbk, unsigned pieces) {
 
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap) {
+                const DeviceTables& tab, hipStream_t s, size_t cap) {
 	if(!n_out) return PPH_OK;
 	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out, cap);
-	if(!ws || sh.total == SIZE_MAX || ws_bytes < sh.total || !wtab) return PPH_E_INVALID;
+	if(!ws || sh.total == SIZE_MAX || ws_bytes < sh.total) return PPH_E_INVALID;
 	if(!blm_planes_fit(sh.kw_count, sh.wc, round_up(sh.r_blk, sh.rt))) return PPH_E_INVALID;
 	uint32_t* cb = static_cast<uint32_t*>(ws);
 	uint32_t* wb = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ws) + sh.cb_bytes);
@@ -651,7 +712,7 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 			                         0u};
 			if(e == hipSuccess)
 				hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(rows, wv / 32, 1), dim3(256), 0, s, cp,
-				                   (const BlmCombinePiece*)nullptr, wtab);
+				                   (const BlmCombinePiece*)nullptr);
 			if(e == hipSuccess) e = hipGetLastError();
 			if(e != hipSuccess) return set_error(e);
 		}
@@ -850,8 +911,8 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	return PPH_OK;
 }
 
-int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTables& tab, hipStream_t s) {
-	if(!b || !wtab) return PPH_E_INVALID;
+int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s) {
+	if(!b) return PPH_E_INVALID;
 	if(b->coef.empty()) return PPH_OK;
 	const BlmCoefJob none_c{};
 	const BlmWordsPiece none_w{};
@@ -870,7 +931,7 @@ int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTable
 				e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
 		if(e == hipSuccess) {
 			hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(w.comb_rows, w.comb_wbk, w.comb_n), dim3(256), 0, s,
-			                   none_m, b->d_comb + w.comb0, wtab);
+			                   none_m, b->d_comb + w.comb0);
 			e = hipGetLastError();
 		}
 	}

@@ -89,7 +89,7 @@ int autotune(struct ::pph_tuning* out);
 
 // Bilinear form (pph_bilinear.hip); int status codes.
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
-int bilinear_prepare(size_t slot_bytes, unsigned n_in, unsigned n_out);
+int bilinear_supported(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
@@ -99,7 +99,7 @@ int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, con
 size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap = 0);
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const uint16_t* wtab, const DeviceTables& tab, hipStream_t s, size_t cap = 0);
+                const DeviceTables& tab, hipStream_t s, size_t cap = 0);
 // Row tile of the bilinear GEMM for n_out outputs: the one that pads the
 // outputs least, a 64-row tile's rows weighed 1.42 (measured: its waves expand
 // ten masks per step where the others expand eight, and issue twice the
@@ -126,9 +126,8 @@ struct BlmBatch;
 bool blm_batch_eligible(const BlmBatchJob& j, size_t ws_budget);
 size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget);
 int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, BlmBatch** out);
-int blm_batch_enqueue(const BlmBatch* b, const uint16_t* wtab, const DeviceTables& tab, hipStream_t s);
+int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s);
 void blm_batch_destroy(BlmBatch* b);
-const uint16_t* bilinear_wtab();  // recombination table on the current device (after bilinear_prepare)
 hipError_t launch_xor_byte(uint8_t* p, uint8_t mask, hipStream_t s);
 hipError_t launch_lds_gather_ceiling(const uint16_t* exp_tab, unsigned blocks, unsigned rounds, uint32_t seed,
                                      uint32_t* sink, hipStream_t s);

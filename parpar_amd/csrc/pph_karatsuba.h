@@ -71,4 +71,42 @@ __device__ __forceinline__ void bl_expand_slices(const uint32_t s[16], uint32_t 
 	}
 }
 
+// Karatsuba recombination, bit-sliced: products p of the components of two
+// N-term polynomials (the order of bl_expand_slices) -> the 2N - 1 coefficients
+// of their product, L + (D + L + H) x^(N/2) + H x^N per level.
+template <unsigned N, unsigned C>
+__device__ __forceinline__ void bl_recombine_n(const uint32_t* p, uint32_t* r) {
+	if constexpr(N == 1) {
+		r[0] = p[0];
+	} else {
+		constexpr unsigned H = N / 2, K = C / 3;
+		uint32_t l[2 * H - 1], h[2 * H - 1], d[2 * H - 1];
+		bl_recombine_n<H, K>(p, l);
+		bl_recombine_n<H, K>(p + K, h);
+		bl_recombine_n<H, K>(p + 2 * K, d);
+#pragma unroll
+		for(unsigned i = 0; i < 2 * N - 1; i++) r[i] = 0;
+#pragma unroll
+		for(unsigned i = 0; i < 2 * H - 1; i++) {
+			r[i] ^= l[i];
+			r[i + H] ^= d[i] ^ l[i] ^ h[i];
+			r[i + N] ^= h[i];
+		}
+	}
+}
+template <unsigned N>
+__device__ __forceinline__ void bl_recombine(const uint32_t* p, uint32_t* r) {
+	bl_recombine_n<N, kBl>(p, r);
+}
+// coefficients 16 .. 30 folded down modulo x^16 + x^12 + x^3 + x + 1
+__device__ __forceinline__ void bl_reduce_slices(uint32_t* r) {
+#pragma unroll
+	for(unsigned b = 30; b >= 16; b--) {
+		r[b - 4] ^= r[b];
+		r[b - 13] ^= r[b];
+		r[b - 15] ^= r[b];
+		r[b - 16] ^= r[b];
+	}
+}
+
 }  // namespace pph

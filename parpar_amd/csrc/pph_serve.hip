@@ -235,7 +235,7 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 		e = launch_slice_hash_to(j->data, j->slot, (unsigned)j->n_in, j->lens, S, j->ifsc, 20, j->ifsc + 16, 20,
 		                         *s->tab, a);
 	if(e == hipSuccess && j->n_out && j->form == PPH_FORM_BILINEAR &&
-	   bilinear_prepare(j->slot, (unsigned)j->n_in, j->n_out) != PPH_OK)
+	   bilinear_supported(j->slot, (unsigned)j->n_in, j->n_out) != PPH_OK)
 		j->form = PPH_FORM_PLAIN;
 	if(e == hipSuccess && j->n_out) {
 		const bool bl = j->form == PPH_FORM_BILINEAR;
