@@ -432,6 +432,23 @@ def main_hashes(args):
                       "rows": rows}))
 
 
+
+def batch_row_split(n_out):
+    """Padded rows of a bilinear job in a batch: tiles of 256, 128 and 64 rows,
+    the cheapest split with 128-row tiles weighed 1.05 and 64-row tiles 1.35
+    (the library's blm_row_split)."""
+    best = None
+    for a in range(-(-n_out // 256) + 1):
+        for b in range(4):
+            for c in range(4):
+                rows = 256 * a + 128 * b + 64 * c
+                if rows < n_out:
+                    continue
+                cost = 256.0 * a + 1.05 * 128 * b + 1.35 * 64 * c + 1e-6 * (a + b + c)
+                if best is None or cost < best[0]:
+                    best = (cost, rows)
+    return best[1]
+
 def main_batched(args):
     import numpy as np
     import parpar_amd
@@ -527,18 +544,16 @@ def main_batched(args):
     # their fp4 MFMA work, 81 components x 2 ops per mul-add, over the fused
     # region's HIP-event time (which also holds the word-plane, recombination,
     # plain and dense launches).  executed_mfma_ops adds the GEMM's padding
-    # (inputs to 256, at least 512; outputs to its 64-, 128- or 256-row tile).
+    # (inputs to 256, at least 512; outputs to the job's row tiles of 256, 128
+    # and 64 rows, split as the library splits them).
     bl_ops = bl_exec = 0
     for k, j in enumerate(jobs):
         if forms[k] != "bilinear":
             continue
         words = j.slot // 2
         kw = max(16, -(-j.n_in // 256) * 8)
-        pads = {256: -(-j.n_out // 256) * 256, 128: -(-j.n_out // 128) * 128, 64: -(-j.n_out // 64) * 64}
-        cost = {256: pads[256], 128: pads[128], 64: 1.42 * pads[64]}  # the library's row-tile rule
-        rt = 256 if cost[256] <= min(cost[128], cost[64]) else (128 if cost[128] <= cost[64] else 64)
         bl_ops += 2 * 81 * j.n_in * j.n_out * words
-        bl_exec += 2 * 81 * (32 * kw) * (-(-j.n_out // rt) * rt) * words
+        bl_exec += 2 * 81 * (32 * kw) * batch_row_split(j.n_out) * words
     achieved_tf = bl_ops / (fused_ms / 1e3) / 1e12
     hbm_eq = 2.0 * ma / (fused_ms / 1e3) / 1e9
     res = {

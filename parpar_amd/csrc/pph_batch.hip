@@ -74,20 +74,21 @@ unsigned tune_bucket(unsigned r) {
 }
 
 // Share of the bilinear form's padded work that is real: its GEMM pads the
-// inputs to a multiple of 256 and the outputs to its row tile (blm_row_tile).
-double bilinear_fill(unsigned n_in, unsigned n_out) {
-	const size_t rt = blm_row_tile(n_out);
-	return (double)n_in / (double)round_up(n_in, 256) * ((double)n_out / (double)round_up(n_out, rt));
+// inputs to a multiple of 256 and the outputs to its row tile (blm_row_tile;
+// in a batch, to the weighed rows of the job's mixed row tiles).
+double bilinear_fill(unsigned n_in, unsigned n_out, bool batched) {
+	const double rows = batched ? blm_row_split(n_out).cost : (double)round_up(n_out, blm_row_tile(n_out));
+	return (double)n_in / (double)round_up(n_in, 256) * ((double)n_out / rows);
 }
 
-double bilinear_rate(const pph_tuning& tu, unsigned n_in, unsigned n_out) {
+double bilinear_rate(const pph_tuning& tu, unsigned n_in, unsigned n_out, bool batched) {
 	const unsigned bk = tune_bucket(n_out);
 	if(tu.version < 3 || !(tu.bilinear[bk] > 0) || !(tu.bilinear_wide[bk] > 0) || !n_in) return 0;
 	// log-linear in the input count between the two measured widths (512, 4096)
 	const double f = std::min(1.0, std::max(0.0, (std::log2((double)n_in) - 9.0) / 3.0));
 	const double v = std::exp((1 - f) * std::log((double)tu.bilinear[bk]) + f * std::log((double)tu.bilinear_wide[bk]));
 	// scaled by the job's padding against that of the measured shapes
-	return v * bilinear_fill(n_in, n_out) / bilinear_fill(512, kTuneR[bk]);
+	return v * bilinear_fill(n_in, n_out, batched) / bilinear_fill(512, kTuneR[bk], batched);
 }
 
 double plain_rate(const pph_tuning& tu, unsigned n_out) {
@@ -396,12 +397,19 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 	// choose the plain grouping for the batch, then the form per job, from the
 	// measured rates (mul-adds per second by recovery-count bucket)
 	auto ma_of = [](const pph_job& jb) { return (double)jb.n_in * jb.n_out * (double)(jb.slot_bytes / 2); };
+	// The plain kernel computes whole groups of G rows (the rows past the
+	// recovery count run with exponent 0 and are not stored), so a job costs
+	// its rows rounded up to G at the rate measured for that many rows.
+	auto plain_time = [&](const pph_job& jb, unsigned G) {
+		const unsigned rows = (jb.n_out + G - 1) / G * G;
+		const unsigned bk = tune_bucket(rows);
+		return ma_of(jb) * ((double)rows / jb.n_out) / std::max(1.0f, G == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
+	};
 	double t8 = 0, t16 = 0;
 	for(const auto& jb : b->jobs) {
 		if(!jb.n_in || !jb.n_out) continue;
-		const unsigned bk = tune_bucket(jb.n_out);
-		t8 += ma_of(jb) / std::max(1.0f, tu.plain_g8[bk]);
-		t16 += ma_of(jb) / std::max(1.0f, tu.plain_g16[bk]);
+		t8 += plain_time(jb, 8);
+		t16 += plain_time(jb, 16);
 	}
 	b->group = t8 < t16 ? 8 : 16;
 	std::vector<double> pred(n_jobs, 0.0);
@@ -411,9 +419,9 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		if(!jb.n_in || !jb.n_out) continue;
 		const unsigned bk = tune_bucket(jb.n_out);
 		const double ma = ma_of(jb);
-		const double tp = ma / std::max(1.0f, b->group == 8 ? tu.plain_g8[bk] : tu.plain_g16[bk]);
+		const double tp = plain_time(jb, b->group);
 		const double td = tu.dense[bk] > 0 ? ma / tu.dense[bk] : 1e30;
-		const double rb = bilinear_rate(tu, jb.n_in, jb.n_out);
+		const double rb = bilinear_rate(tu, jb.n_in, jb.n_out, true);
 		const double tb = rb > 0 ? ma / rb : 1e30;
 		double best = tp;
 		if(td < best) {

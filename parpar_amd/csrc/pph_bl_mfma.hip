@@ -153,14 +153,17 @@ __global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmW
 		const unsigned w0 = 512 * (unsigned)(u - (size_t)kw * per_kw);
 		uint32_t a[32];
 		if(w0 < P.wv) {
-			const uint8_t* in = P.in + 2 * (size_t)w0 + (size_t)kw * 32 * P.slot_bytes;
-			const unsigned n = P.n_in > kw * 32 ? P.n_in - kw * 32 : 0;  // inputs of this group
+			// The slot pointer and the group's input count are made wave-uniform
+			// explicitly: a descriptor the compiler takes for per-lane is issued in
+			// a loop over lanes (one per load).
+			const uint8_t* in = blm_uniform(P.in + 2 * (size_t)w0 + (size_t)kw * 32 * P.slot_bytes);
+			const unsigned n = __builtin_amdgcn_readfirstlane(P.n_in > kw * 32 ? P.n_in - kw * 32 : 0);
 #pragma unroll
 			for(unsigned k = 0; k < 32; k++) {
 				// past the last input: an empty resource, whose loads return 0
+				const uint8_t* src = blm_uniform(in + k * P.slot_bytes);
 				a[4 * (k % 8) + k / 8] = __builtin_amdgcn_raw_buffer_load_b32(
-				    __builtin_amdgcn_make_buffer_rsrc((void*)in, 0, k < n ? 2048 : 0, 0x00020000), (int)(4 * tid), 0, 0);
-				in += P.slot_bytes;
+				    __builtin_amdgcn_make_buffer_rsrc((void*)src, 0, k < n ? 2048 : 0, 0x00020000), (int)(4 * tid), 0, 0);
 			}
 		} else {
 #pragma unroll
@@ -195,7 +198,8 @@ struct BlmPiece {
 	uint32_t* par;       // parity bits [t][wc_pad / 32][r_pad]
 	uint32_t steps;      // 64-input steps, a multiple of 8
 	uint32_t wc_pad;     // words, a multiple of 512
-	uint32_t r_pad;      // outputs, a multiple of the row tile
+	uint32_t r_pad;      // outputs (the planes' row stride)
+	uint32_t r_base;     // first output of the piece's row tiles (a multiple of 64)
 	uint32_t w_tiles, r_tiles;
 	uint32_t item0, items;
 	uint32_t row_fast;
@@ -267,7 +271,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		           P.par + t * pp,
 		           (uint32_t)(pp * 4),
 		           wt * WT,
-		           rt * RT,
+		           P.r_base + rt * RT,
 		           P.steps,
 		           P.wc_pad,
 		           P.r_pad};
@@ -701,8 +705,8 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 			                   (const BlmWordsPiece*)nullptr);
 			const unsigned w_tiles = (unsigned)(wc / (65536 / sh.rt)), r_tiles = (unsigned)(r_pad / sh.rt);
 			const unsigned items = kBl * w_tiles * r_tiles;
-			const BlmPiece piece{wb, cb, par, (uint32_t)(sh.kw_count / 2), (uint32_t)wc, (uint32_t)r_pad, w_tiles, r_tiles,
-			                     0u, items, sh.row_fast};
+			const BlmPiece piece{wb, cb, par, (uint32_t)(sh.kw_count / 2), (uint32_t)wc, (uint32_t)r_pad, 0u, w_tiles,
+			                     r_tiles, 0u, items, sh.row_fast};
 			e = hipGetLastError();
 			if(e == hipSuccess) e = blm_launch_gemm((unsigned)sh.rt, piece, nullptr, items, tab, s);
 			const BlmCombinePiece cp{par,           (uint8_t*)out_slots + r0 * slot_bytes + 2 * w0,
@@ -729,6 +733,14 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 // per row tile over a cross-job item table, and one recombination launch.
 // Pieces run longest first (input count descending), so that a wave's last
 // items are its shortest.
+//
+// Overlapped form (two halves of the workspace, waves alternating between
+// them): the GEMMs run back to back on the caller's stream while a second
+// stream writes the next wave's word planes and recombines the previous
+// wave's parity bits.  Those kernels are memory-bound and the GEMM is
+// MFMA-bound, and they fit on a CU beside a GEMM workgroup (which holds one
+// wave of ~390 registers per SIMD): the word planes run as about one
+// workgroup per CU for that reason.
 struct BlmBatch {
 	std::vector<BlmCoefJob> coef;
 	struct Wave {
@@ -745,20 +757,31 @@ struct BlmBatch {
 	BlmWordsPiece* d_words = nullptr;
 	BlmPiece* d_gemm = nullptr;
 	BlmCombinePiece* d_comb = nullptr;
+	bool overlap = false;
+	hipStream_t side = nullptr;
+	hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_words[2] = {}, ev_gemm[2] = {};
 };
 
 namespace {
 
+// A job's rows are covered by row tiles of several heights (blm_row_split:
+// 256-row tiles first, then 128 and 64), each height in its own GEMM launch;
+// plane rows are multiples of the largest word tile used.
 struct BlmJobShape {
-	size_t kw_count, rt, wt, r_pad, slot_words, cb_bytes;  // wt: word tile, plane rows are multiples
+	size_t kw_count, wt, r_pad, slot_words, cb_bytes;  // wt: word tile, plane rows are multiples
+	unsigned tiles[3];                                 // row tiles of 256, 128, 64 rows
 };
 
 BlmJobShape blm_job_shape(const BlmBatchJob& j) {
 	BlmJobShape s{};
 	s.kw_count = std::max<size_t>(kBlmMinKw, round_up(j.n_in, kBlmKStep) / 32);
-	s.rt = blm_row_tile(j.n_out);
-	s.wt = std::max<size_t>(kBlmWordTile, 65536 / s.rt);
-	s.r_pad = round_up(j.n_out, s.rt);
+	const BlmRowSplit rs = blm_row_split(j.n_out);
+	s.wt = kBlmWordTile;
+	for(unsigned k = 0; k < 3; k++) {
+		s.tiles[k] = rs.n[k];
+		if(rs.n[k]) s.wt = std::max<size_t>(s.wt, 65536 / (256u >> k));
+	}
+	s.r_pad = rs.rows;
 	s.slot_words = j.slot_bytes / 2;
 	s.cb_bytes = round_up(kBl * s.kw_count * s.r_pad * 4, 256);
 	return s;
@@ -802,16 +825,20 @@ size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget
 	}
 	if(!n) return 0;
 	// every piece at once when that fits, else the budget (at least the
-	// coefficients and one piece)
-	return std::max(cb + big, std::min(cb + all, ws_budget));
+	// coefficients and one piece, two for the overlapped form)
+	return std::max(cb + (all > big ? 2 : 1) * big, std::min(cb + all, ws_budget));
 }
 
 void blm_batch_destroy(BlmBatch* b) {
 	if(!b) return;
+	if(b->side) (void)hipStreamSynchronize(b->side);
 	if(b->d_coef) (void)hipFree(b->d_coef);
 	if(b->d_words) (void)hipFree(b->d_words);
 	if(b->d_gemm) (void)hipFree(b->d_gemm);
 	if(b->d_comb) (void)hipFree(b->d_comb);
+	for(hipEvent_t e : {b->ev_start, b->ev_end, b->ev_words[0], b->ev_words[1], b->ev_gemm[0], b->ev_gemm[1]})
+		if(e) (void)hipEventDestroy(e);
+	if(b->side) (void)hipStreamDestroy(b->side);
 	delete b;
 }
 
@@ -843,7 +870,13 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			pieces.push_back(Piece{k, w0, std::min(pw, sh[k].slot_words - w0)});
 	std::stable_sort(pieces.begin(), pieces.end(),
 	                 [&](const Piece& x, const Piece& y) { return sh[x.job].kw_count > sh[y.job].kw_count; });
-	const size_t area = ws_bytes - off;
+	// Overlapped form: waves alternate between two halves of the piece area
+	// (when each half holds the largest piece and there are two pieces or more).
+	size_t area = ws_bytes - off, big = 0;
+	for(const Piece& pc : pieces) big = std::max(big, blm_piece_bytes(sh[pc.job], pc.wc));
+	const char* ov = getenv("PPH_BATCH_OVERLAP");
+	b->overlap = pieces.size() >= 2 && area / 2 / 256 * 256 >= big && !(ov && ov[0] == '0');
+	if(b->overlap) area = area / 2 / 256 * 256;
 	std::vector<BlmWordsPiece> words;
 	std::vector<BlmPiece> gemm[3];
 	std::vector<BlmCombinePiece> comb;
@@ -854,6 +887,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 		w.words0 = (unsigned)words.size();
 		w.comb0 = (unsigned)comb.size();
 		for(auto& g : gemm) g.clear();
+		const size_t half = b->overlap ? (b->waves.size() % 2) * area : 0;
 		size_t used = 0;
 		for(; p < pieces.size(); p++) {
 			const Piece& pc = pieces[p];
@@ -865,19 +899,24 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 				if(!used) return PPH_E_INVALID;  // a piece larger than the area
 				break;
 			}
-			uint32_t* wbp = reinterpret_cast<uint32_t*>(base + off + used);
-			uint32_t* par = reinterpret_cast<uint32_t*>(base + off + used + wb_bytes);
+			uint32_t* wbp = reinterpret_cast<uint32_t*>(base + off + half + used);
+			uint32_t* par = reinterpret_cast<uint32_t*>(base + off + half + used + wb_bytes);
 			used += need;
 			const BlmBatchJob& j = jobs[pc.job];
 			words.push_back(BlmWordsPiece{wbp, (const uint8_t*)j.in_slots + 2 * pc.w0, j.slot_bytes,
 			                              (uint32_t)s.kw_count, (uint32_t)wc, (uint32_t)pc.wc, j.n_in});
 			w.words_threads = std::max(w.words_threads, s.kw_count * wc / 2);
-			const unsigned ti = s.rt == 256 ? 0 : s.rt == 128 ? 1 : 2;
-			const unsigned w_tiles = (unsigned)(wc / (65536 / s.rt)), r_tiles = (unsigned)(s.r_pad / s.rt);
-			const unsigned items = kBl * w_tiles * r_tiles;
-			gemm[ti].push_back(BlmPiece{wbp, cb[pc.job], par, (uint32_t)(s.kw_count / 2), (uint32_t)wc,
-			                            (uint32_t)s.r_pad, w_tiles, r_tiles, w.items[ti], items, 1u});
-			w.items[ti] += items;
+			unsigned r_base = 0;
+			for(unsigned ti = 0; ti < 3; ti++) {
+				const unsigned rt = 256u >> ti, r_tiles = s.tiles[ti];
+				if(!r_tiles) continue;
+				const unsigned w_tiles = (unsigned)(wc / (65536 / rt));
+				const unsigned items = kBl * w_tiles * r_tiles;
+				gemm[ti].push_back(BlmPiece{wbp, cb[pc.job], par, (uint32_t)(s.kw_count / 2), (uint32_t)wc,
+				                            (uint32_t)s.r_pad, r_base, w_tiles, r_tiles, w.items[ti], items, 1u});
+				w.items[ti] += items;
+				r_base += rt * r_tiles;
+			}
 			comb.push_back(BlmCombinePiece{par, (uint8_t*)j.out_slots + 2 * pc.w0, j.slot_bytes, (uint32_t)s.r_pad,
 			                               (uint32_t)(pc.wc / 32), (uint32_t)(wc / 32), j.n_out, 0u, 0u});
 			w.comb_rows = std::max<size_t>(w.comb_rows, j.n_out);
@@ -903,6 +942,11 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	if(e == hipSuccess) e = upload(&b->d_words, words);
 	if(e == hipSuccess) e = upload(&b->d_gemm, gemm_all);
 	if(e == hipSuccess) e = upload(&b->d_comb, comb);
+	if(b->overlap) {
+		if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+		for(hipEvent_t* ev : {&b->ev_start, &b->ev_end, &b->ev_words[0], &b->ev_words[1], &b->ev_gemm[0], &b->ev_gemm[1]})
+			if(e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+	}
 	if(e != hipSuccess) {
 		blm_batch_destroy(b.release());
 		return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);
@@ -921,6 +965,48 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 	hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(b->coef_threads, tab), (unsigned)b->coef.size()), dim3(256), 0, s,
 	                   none_c, b->d_coef, tab.exp);
 	hipError_t e = hipGetLastError();
+	if(b->overlap) {
+		// Caller's stream: GEMM(k) after the word planes of wave k.  Side stream:
+		// words(0); then per wave k: words(k + 1) (its half was last read by
+		// GEMM(k - 1) and recombination(k - 1), both before it in this order),
+		// recombination(k) after GEMM(k).  The side stream joins back at the end.
+		const unsigned cus = (unsigned)(tab.cus > 0 ? tab.cus : 256);
+		// wave 0 runs alone (every CU); later waves run beside a GEMM (a
+		// workgroup per CU)
+		auto words = [&](const BlmBatch::Wave& w) {
+			const unsigned full = blm_blocks(w.words_threads, tab);
+			const unsigned gx = &w == &b->waves[0] ? full : std::min(full, std::max(1u, (cus + w.words_n - 1) / w.words_n));
+			hipLaunchKernelGGL(k_blm_words, dim3(gx, w.words_n), dim3(256), 0, b->side, none_w, b->d_words + w.words0);
+			return hipGetLastError();
+		};
+		if(e == hipSuccess) e = hipEventRecord(b->ev_start, s);
+		if(e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_start, 0);
+		if(e == hipSuccess) e = words(b->waves[0]);
+		if(e == hipSuccess) e = hipEventRecord(b->ev_words[0], b->side);
+		for(size_t k = 0; k < b->waves.size() && e == hipSuccess; k++) {
+			const BlmBatch::Wave& w = b->waves[k];
+			e = hipStreamWaitEvent(s, b->ev_words[k % 2], 0);
+			for(unsigned ti = 0; ti < 3 && e == hipSuccess; ti++)
+				if(w.gemm_n[ti])
+					e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
+			if(e == hipSuccess) e = hipEventRecord(b->ev_gemm[k % 2], s);
+			if(e == hipSuccess && k + 1 < b->waves.size()) {
+				e = words(b->waves[k + 1]);
+				if(e == hipSuccess) e = hipEventRecord(b->ev_words[(k + 1) % 2], b->side);
+			}
+			if(e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_gemm[k % 2], 0);
+			if(e == hipSuccess) {
+				hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(w.comb_rows, w.comb_wbk, w.comb_n), dim3(256), 0,
+				                   b->side, none_m, b->d_comb + w.comb0);
+				e = hipGetLastError();
+			}
+		}
+		// join even after an error, so that a capture on `s` can end
+		const hipError_t j = hipEventRecord(b->ev_end, b->side);
+		const hipError_t j2 = j == hipSuccess ? hipStreamWaitEvent(s, b->ev_end, 0) : j;
+		if(e == hipSuccess) e = j2;
+		return e == hipSuccess ? PPH_OK : set_error(e);
+	}
 	for(const BlmBatch::Wave& w : b->waves) {
 		if(e != hipSuccess) break;
 		hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(w.words_threads, tab), w.words_n), dim3(256), 0, s, none_w,

@@ -83,7 +83,8 @@ hipError_t launch_file_md5(const uint8_t* slots, size_t slot_bytes, unsigned cou
 // Spans of file MD5 chains (n Md5Span records, device-resident), one wave each.
 hipError_t launch_file_md5_round(const void* spans, unsigned n, hipStream_t s);
 // Rate model of the autotuner's table (pph_batch.hip): mul-adds per second.
-double bilinear_rate(const struct ::pph_tuning& tu, unsigned n_in, unsigned n_out);  // 0: unavailable
+// 0: unavailable; batched: the padding of the batch's mixed row tiles
+double bilinear_rate(const struct ::pph_tuning& tu, unsigned n_in, unsigned n_out, bool batched = false);
 double plain_rate(const struct ::pph_tuning& tu, unsigned n_out);
 int autotune(struct ::pph_tuning* out);
 
@@ -109,6 +110,31 @@ inline size_t blm_row_tile(size_t n_out) {
 	const double c256 = (double)ru(n_out, 256), c128 = (double)ru(n_out, 128), c64 = 1.42 * (double)ru(n_out, 64);
 	if(c256 <= c128 && c256 <= c64) return 256;
 	return c128 <= c64 ? 128 : 64;
+}
+
+// Rows of one job in the batched form, covered by tiles of several heights:
+// n[0] tiles of 256 rows, then n[1] of 128, then n[2] of 64, the split that
+// costs least with a row of the 128-row tile weighed 1.05 and of the 64-row
+// tile 1.35 against the 256-row tile's (measured per call on 768..3072
+// inputs: 1.00-1.08 and 1.22-1.41), fewer tiles on ties.  Returns the padded
+// row count.
+struct BlmRowSplit {
+	unsigned n[3];
+	size_t rows;
+	double cost;
+};
+inline BlmRowSplit blm_row_split(size_t n_out) {
+	BlmRowSplit best{{0, 0, 0}, 0, 1e300};
+	const size_t top = (n_out + 255) / 256;
+	for(size_t a = 0; a <= top; a++)
+		for(unsigned b = 0; b < 4; b++)
+			for(unsigned c = 0; c < 4; c++) {
+				const size_t rows = 256 * a + 128 * b + 64 * c;
+				if(rows < n_out) continue;
+				const double cost = 256.0 * a + 1.05 * 128 * b + 1.35 * 64 * c + 1e-6 * (a + b + c);
+				if(cost < best.cost) best = BlmRowSplit{{(unsigned)a, b, c}, rows, cost};
+			}
+	return best;
 }
 
 // Many bilinear jobs as one sequence of launches (pph_bl_mfma.hip): waves of

@@ -20,6 +20,8 @@ SPECS = [
     (130, 20000, list(range(100, 117)), 7),
     (3, 8192, list(range(0, 64000, 2000)), 8192),
     (19, 2, [0, 9], 1),
+    # 400 rows: in the bilinear form's batch, row tiles of 256, 128 and 64 rows
+    (21, 3000, list(range(400)), 2999),
 ]
 
 
