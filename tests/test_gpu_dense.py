@@ -144,11 +144,12 @@ def test_bilinear_chunked(monkeypatch):
     lib().pph_bilinear_release()
 
 
-@pytest.mark.parametrize("tile", ["128", "256"])
+@pytest.mark.parametrize("tile", ["64", "128", "256"])
 @pytest.mark.parametrize("n,n_out", [(257, 300), (40, 129), (520, 256)])
 def test_bilinear_row_tiles(monkeypatch, tile, n, n_out):
-    """Both GEMM row tiles (128 outputs x 512 words, 256 x 256) give the plain
-    form's bytes, with outputs and inputs off the tile and step sizes."""
+    """Every GEMM row tile (64 outputs x 1024 words, 128 x 512, 256 x 256)
+    gives the plain form's bytes, with outputs and inputs off the tile and step
+    sizes."""
     say("bilinear row tile", tile, n, n_out)
     S = 5000
     rng = np.random.default_rng(n + n_out)
