@@ -43,6 +43,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_batch(const BatchJob* 
                                                                 unsigned* __restrict__ counter,
                                                                 const uint16_t* __restrict__ exp_tab) {
 	__shared__ uint16_t s_exp[65536];
+	__shared__ uint16_t s_lin[kPlainThreads];
 	__shared__ unsigned s_item;
 	load_exp_table(s_exp, exp_tab);
 	for(;;) {
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_batch(const BatchJob* 
 		if(k >= n_items) break;
 		const BatchItem it = items[k];
 		const BatchJob jb = jobs[it.job];
-		plain_tile<G, false>(s_exp, jb.in, jb.slot_words, jb.n_in, jb.in_logs, jb.exps, nullptr, 0, jb.out, jb.n_out,
+		plain_tile<G, false>(s_exp, s_lin, jb.in, jb.slot_words, jb.n_in, jb.in_logs, jb.exps, nullptr, 0, jb.out, jb.n_out,
 		                     0, it.col, it.grp);
 	}
 }

@@ -355,6 +355,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
     uint16_t* __restrict__ out, unsigned n_out, int accumulate, const uint16_t* __restrict__ exp_tab,
     unsigned col_tiles, unsigned groups) {
 	__shared__ uint16_t s_exp[65536];
+	__shared__ uint16_t s_lin[kPlainThreads];
 	load_exp_table(s_exp, exp_tab);
 	__syncthreads();
 
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
 	if((total & 7u) == 0) L = (blockIdx.x & 7u) * (total >> 3) + (blockIdx.x >> 3);
 	const unsigned col = L / groups;
 	const unsigned grp = L - col * groups;
-	plain_tile<G, TABLE>(s_exp, in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, col,
+	plain_tile<G, TABLE>(s_exp, s_lin, in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, col,
 	                     grp);
 }
 

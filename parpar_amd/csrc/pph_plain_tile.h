@@ -33,8 +33,11 @@ __device__ __forceinline__ void load_exp_table(uint16_t* s_exp, const uint16_t* 
 }
 
 // One work item: column tile `col` (8192 words) x recovery slices [G*grp, G*grp+G).
+// s_lin: LDS for kPlainThreads input logs.  Every thread of the workgroup
+// calls it (it synchronises the workgroup).
 template <int G, bool TABLE>
-__device__ __forceinline__ void plain_tile(const uint16_t* s_exp, const uint16_t* __restrict__ in, size_t slot_words,
+__device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_lin, const uint16_t* __restrict__ in,
+                                           size_t slot_words,
                                            unsigned n_in, const uint16_t* __restrict__ in_logs,
                                            const uint16_t* __restrict__ exps, const uint32_t* __restrict__ coef,
                                            unsigned coef_stride, uint16_t* __restrict__ out, unsigned n_out,
@@ -59,9 +62,24 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, const uint16_t
 		}
 	}
 
+	// The next input's words are loaded one input ahead, and the input logs
+	// come from LDS (staged kPlainThreads at a time), so that no wave waits on
+	// a global load at the top of its input loop (a log read from global memory
+	// into a scalar register waited for every load in flight).
+	const uint4 kNone = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+	uint4 v_next = kNone;
+	if(n_in && active) v_next = *reinterpret_cast<const uint4*>(in + w0);
 	for(unsigned i = 0; i < n_in; i++) {
-		uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-		if(active) v = *reinterpret_cast<const uint4*>(in + (size_t)i * slot_words + w0);
+		const uint4 v = v_next;
+		if(i + 1 < n_in && active) v_next = *reinterpret_cast<const uint4*>(in + (size_t)(i + 1) * slot_words + w0);
+		if constexpr(!TABLE) {
+			if(i % kPlainThreads == 0) {
+				__syncthreads();  // the previous chunk has been read by every wave
+				const unsigned k = i + threadIdx.x;
+				s_lin[threadIdx.x] = k < n_in ? in_logs[k] : 0;
+				__syncthreads();
+			}
+		}
 		const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
 		uint32_t lx[8], lxm[8];
 #pragma unroll
@@ -73,7 +91,7 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, const uint16_t
 			lxm[2 * j + 1] = lx[2 * j + 1] - kZeroIdx2;
 		}
 		uint32_t lin = 0;
-		if constexpr(!TABLE) lin = in_logs[i];
+		if constexpr(!TABLE) lin = __builtin_amdgcn_readfirstlane(s_lin[i % kPlainThreads]);
 #pragma unroll
 		for(int r = 0; r < G; r++) {
 			uint32_t lc;
