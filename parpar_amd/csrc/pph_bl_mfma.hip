@@ -57,16 +57,19 @@ __device__ __forceinline__ uint4 blm_nibbles_c(uint32_t m) {
 // A descriptor read from a table, made wave-uniform (scalar registers): the
 // tables are read with vector loads (the kernels also store to global memory),
 // which the compiler would otherwise treat as per-lane values.
+// (Word-wise bit casts: a byte-wise copy through memcpy made the compiler keep
+// the fields as separate bytes in scalar registers and reassemble them on
+// every use, ~100 instructions per GEMM item.)
 template <class T>
 __device__ __forceinline__ T blm_uniform(const T& x) {
 	static_assert(sizeof(T) % 4 == 0, "descriptor words");
-	uint32_t w[sizeof(T) / 4];
-	memcpy(w, &x, sizeof(w));
+	struct Words {
+		uint32_t w[sizeof(T) / 4];
+	};
+	Words w = __builtin_bit_cast(Words, x);
 #pragma unroll
-	for(unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
-	T y;
-	memcpy(&y, w, sizeof(w));
-	return y;
+	for(unsigned k = 0; k < sizeof(T) / 4; k++) w.w[k] = __builtin_amdgcn_readfirstlane(w.w[k]);
+	return __builtin_bit_cast(T, w);
 }
 
 // Coefficient planes of one job's output block: outputs r0 .. r0 + n_rows
