@@ -402,9 +402,12 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	// Before the first item the resource is empty, so its stores are dropped.
 	// The destination is made wave-uniform where it is used (a loop-carried
 	// value the compiler would otherwise treat as per-lane, looping over lanes).
-	uint32_t po[NA * NB];
+	// po[i * NB / 2 + p]: lanes 0..31 hold the mask of block (i, 2p), lanes
+	// 32..63 that of block (i, 2p + 1) (adjacent 32-output blocks: one 256-byte
+	// store per pair)
+	uint32_t po[NA * NB / 2];
 #pragma unroll
-	for(unsigned q = 0; q < NA * NB; q++) po[q] = 0;
+	for(unsigned q = 0; q < NA * NB / 2; q++) po[q] = 0;
 	struct Dst {
 		uint32_t* base;  // parity plane of the item's component
 		uint32_t bytes;  // its size (0: no item)
@@ -418,9 +421,9 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 #pragma unroll
 		for(unsigned i = 0; i < NA; i++)
 #pragma unroll
-			for(unsigned j = 0; j < NB; j++)
-				__builtin_amdgcn_raw_buffer_store_b32(po[i * NB + j], rs, (int)(row32 * 4),
-				                                      d.sp + (int)((i * d.ldb + 32 * j) * 4), 0);
+			for(unsigned p = 0; p < NB / 2; p++)
+				__builtin_amdgcn_raw_buffer_store_b32(po[i * NB / 2 + p], rs, (int)(lane * 4),
+				                                      d.sp + (int)((i * d.ldb + 64 * p) * 4), 0);
 	};
 
 	constexpr int kScale = 116;  // e8m0 2^-11 on both operands
@@ -507,7 +510,8 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		// words m = 8(v / 4) + 4h + v % 4 of each block; gathering the low bytes
 		// of sums c, 4 + c, 8 + c, 12 + c into one dword and shifting their bit 0
 		// up by 4h + c puts word m at bit m of the block's mask.  The two
-		// half-waves hold disjoint bits of the same mask and store the same word.
+		// half-waves hold disjoint bits of the same mask; one lane swap per pair
+		// of blocks joins them.
 		// The sums are read with an AGPR-constrained move, which keeps the
 		// accumulators in AGPRs across the item loop (a plain VALU read inside
 		// the loop makes the allocator move them to VGPRs and spill).  A VALU
@@ -520,19 +524,28 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 #pragma unroll
 		for(unsigned i = 0; i < NA; i++)
 #pragma unroll
-			for(unsigned j = 0; j < NB; j++) {
-				uint32_t u[16];
+			for(unsigned p = 0; p < NB / 2; p++) {
+				uint32_t m2[2];
 #pragma unroll
-				for(int v = 0; v < 16; v++) asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
-				uint32_t m = 0;
+				for(unsigned q = 0; q < 2; q++) {
+					const unsigned j = 2 * p + q;
+					uint32_t u[16];
 #pragma unroll
-				for(int c = 0; c < 4; c++) {
-					const uint32_t w = __builtin_amdgcn_perm(u[4 + c], u[c], 0x0c0c0400u) |
-					                   __builtin_amdgcn_perm(u[12 + c], u[8 + c], 0x04000c0cu);
-					m |= (w & 0x01010101u) << (4 * half + c);
+					for(int v = 0; v < 16; v++) asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(u[v]) : "a"(acc[i][j][v]));
+					uint32_t m = 0;
+#pragma unroll
+					for(int c = 0; c < 4; c++) {
+						const uint32_t w = __builtin_amdgcn_perm(u[4 + c], u[c], 0x0c0c0400u) |
+						                   __builtin_amdgcn_perm(u[12 + c], u[8 + c], 0x04000c0cu);
+						m |= (w & 0x01010101u) << (4 * half + c);
+					}
+					m2[q] = m;
 				}
-				const auto sw = __builtin_amdgcn_permlane32_swap(m, m, false, false);
-				po[i * NB + j] = sw[0] | sw[1];
+				// lanes 32..63 of the first with lanes 0..31 of the second: the OR
+				// joins each block's two half-waves, block 2p in lanes 0..31 and
+				// block 2p + 1 in lanes 32..63
+				const auto sw = __builtin_amdgcn_permlane32_swap(m2[0], m2[1], false, false);
+				po[i * NB / 2 + p] = sw[0] | sw[1];
 			}
 		pd = Dst{cur.pbase, cur.pbytes, sp, cur.ldb};
 		cur = nxt;
