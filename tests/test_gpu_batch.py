@@ -120,17 +120,22 @@ def test_autotune_rates_are_sane():
     assert d["plain_g16"][4] > d["plain_g16"][0]
 
 
-def test_batch_bilinear_in_waves(refs, monkeypatch):
+@pytest.mark.parametrize("overlap", ["1", "0"])
+def test_batch_bilinear_in_waves(refs, monkeypatch, overlap):
     """The bilinear jobs' cross-job plan under a small workspace: 1024-word
-    pieces (53 of them) in waves of a few, each wave one word-plane launch, one
-    GEMM launch over the pieces' items and one recombination launch; eager and
-    graph replays equal the oracle."""
+    pieces in waves of a few, each wave one word-plane launch, one GEMM launch
+    per row-tile height over the pieces' items and one recombination launch,
+    pipelined on two streams over two halves of the workspace (overlap "1") or
+    in sequence on one (PPH_BATCH_OVERLAP=0); eager and graph replays equal the
+    oracle."""
     monkeypatch.setenv("PPH_BATCH_BILINEAR_WORKSPACE", str(40 << 20))
     monkeypatch.setenv("PPH_BATCH_PIECE_WORDS", "1024")
+    monkeypatch.setenv("PPH_BATCH_OVERLAP", overlap)
     jobs = make_jobs()
     batch = B.Batch(jobs, tuning=B.fixed_tuning(plain=0.001, dense=0.002, bilinear=1.0))
     monkeypatch.delenv("PPH_BATCH_BILINEAR_WORKSPACE")
     monkeypatch.delenv("PPH_BATCH_PIECE_WORDS")
+    monkeypatch.delenv("PPH_BATCH_OVERLAP")
     assert all(batch.plan(j)["form"] == "bilinear" for j in range(len(jobs)))
     for use_graph in (False, True, True):
         fill_all(jobs)
