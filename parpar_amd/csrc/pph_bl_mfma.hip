@@ -787,7 +787,7 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 // stream writes the next wave's word planes and recombines the previous
 // wave's parity bits.  Those kernels are memory-bound and the GEMM is
 // MFMA-bound, and they fit on a CU beside a GEMM workgroup (which holds one
-// wave of ~390 registers per SIMD): the word planes run as about one
+// wave of ~380 registers per SIMD): the word planes run as about one
 // workgroup per CU for that reason.
 struct BlmBatch {
 	std::vector<BlmCoefJob> coef;
