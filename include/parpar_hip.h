@@ -257,6 +257,15 @@ int pph_muladd_dense(const void* in_slots, size_t slot_bytes, unsigned n_in, con
  * whatever streams they use. */
 int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                         const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate, void* stream);
+/* The bilinear form with explicit coefficients (decoding, repair): the mul-add
+ * that IPAR2ProcBackend::addInput(buf, size, const uint16_t* coeffs, flush)
+ * feeds (gf16/controller.h:170; PAR2ProcCPU::set_coeffs' explicit overload,
+ * gf16/controller_cpu.cpp:372-385).  coef_log2[i * coef_stride + r] =
+ * 2*log2(c), 0xFFFFFFFF for c == 0 (pph_muladd_plain's table), coef_stride >=
+ * n_out; data-form input slots; workspace and stream behaviour as
+ * pph_muladd_bilinear. */
+int pph_muladd_bilinear_coef(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint32_t* coef_log2,
+                             unsigned coef_stride, void* out_slots, unsigned n_out, int accumulate, void* stream);
 int pph_bilinear_workspace(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t* bytes);
 int pph_bilinear_release(void);
 /* No counterpart in the reference: for callers that cannot allocate device

@@ -125,6 +125,7 @@ _SIGS = {
     "pph_proc_packet_md5_update": (_i, [_vp, _vp]),
     "pph_verify_slots": (_i, [_vp, _sz, _u, _sz, _u, _vp, _vp]),
     "pph_muladd_bilinear": (_i, [_vp, _sz, _u, _vp, _vp, _vp, _u, _i, _vp]),
+    "pph_muladd_bilinear_coef": (_i, [_vp, _sz, _u, _vp, _u, _vp, _u, _i, _vp]),
     "pph_bilinear_workspace": (_i, [_sz, _u, _u, ctypes.POINTER(_sz)]),
     "pph_bilinear_release": (_i, []),
     "pph_bilinear_mul": (_u16, [_u16, _u16]),

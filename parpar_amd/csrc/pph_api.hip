@@ -720,10 +720,10 @@ int submit_area(pph_proc* p, unsigned area_idx, unsigned n) {
 	uint16_t* logs;
 	meta_ptrs(p, a, &lens, &slots, &logs);
 	const size_t meta = (size_t)p->batch * 3 * sizeof(uint32_t);
-	// the bilinear form takes data-form inputs; batches with explicit
-	// coefficients always run the plain form, and so does a batch whose
-	// bilinear workspace cannot be allocated (the same bytes, more time)
-	bool bl = bilinear_batches(p) && !a.explicit_coef;
+	// the bilinear form takes data-form inputs (numbered or explicit
+	// coefficients alike); a batch whose bilinear workspace cannot be
+	// allocated runs the plain form (the same bytes, more time)
+	bool bl = bilinear_batches(p);
 	if(bl && a.filled_form != 1) {
 		const int rc = bilinear_reserve(p->slot_bytes, n, (unsigned)p->exps.size());
 		if(rc == PPH_E_NOMEM)
@@ -747,8 +747,8 @@ int submit_area(pph_proc* p, unsigned area_idx, unsigned n) {
 	PPH_HIP(hipStreamWaitEvent(p->s_comp, p->ev_copied, 0));
 	if(bl) {
 		const uint16_t* d_logs = reinterpret_cast<const uint16_t*>(a.d_meta + 2 * p->batch);
-		const int rc = pph_muladd_bilinear(a.d_in, p->slot_bytes, n, d_logs, p->d_exps, p->d_out,
-		                                   (unsigned)p->exps.size(), p->processing_add ? 1 : 0, p->s_comp);
+		const int rc = bilinear_call(a.d_in, p->slot_bytes, n, d_logs, p->d_exps, a.explicit_coef ? a.d_coef : nullptr,
+		                             p->r_pad, p->d_out, (unsigned)p->exps.size(), p->processing_add ? 1 : 0, p->s_comp);
 		if(rc) return rc;
 		p->processing_add = true;
 	} else if(!p->exps.empty()) {

@@ -119,10 +119,10 @@ int bilinear_supported(size_t slot_bytes, unsigned n_in, unsigned n_out) {
 // bilinear_workspace_bytes.
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                     const DeviceTables& tab, hipStream_t s) {
+                     const DeviceTables& tab, hipStream_t s, const uint32_t* coef, unsigned coef_stride) {
 	if(!n_out) return PPH_OK;
 	return blm_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate, ws, ws_bytes, tab,
-	                   s);
+	                   s, 0, coef, coef_stride);
 }
 
 }  // namespace pph
@@ -281,12 +281,13 @@ extern "C" int pph_debug_bilinear_workspace_limit(size_t bytes) {
 	return PPH_OK;
 }
 
-extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
-                                   const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate,
-                                   void* stream) {
+namespace pph {
+int bilinear_call(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                  const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, void* out_slots, unsigned n_out,
+                  int accumulate, hipStream_t stream) {
 	if(!n_out) return PPH_OK;
-	if(!in_slots || !out_slots || !input_logs || !exps || !n_in || n_in > 32768 || slot_bytes % kSlotAlign)
-		return PPH_E_INVALID;
+	if(!in_slots || !out_slots || !n_in || n_in > 32768 || slot_bytes % kSlotAlign) return PPH_E_INVALID;
+	if(coef ? coef_stride < n_out : (!input_logs || !exps)) return PPH_E_INVALID;
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if(e != hipSuccess) return set_error(e);
@@ -316,10 +317,27 @@ extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsi
 		}
 	}
 	rc = bilinear_enqueue(in_slots, slot_bytes, n_in, input_logs, exps, out_slots, n_out, accumulate != 0, ws, ws_bytes,
-	                      *tab, (hipStream_t)stream);
+	                      *tab, (hipStream_t)stream, coef, coef_stride);
 	std::lock_guard<std::mutex> lk(d->mu);
 	e = hipEventRecord(d->tail, (hipStream_t)stream);
 	if(e != hipSuccess) return set_error(e);
 	d->tail_valid = true;
 	return rc;
+}
+}  // namespace pph
+
+extern "C" int pph_muladd_bilinear(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                                   const uint16_t* exps, void* out_slots, unsigned n_out, int accumulate,
+                                   void* stream) {
+	if(n_out && (!input_logs || !exps)) return PPH_E_INVALID;
+	return bilinear_call(in_slots, slot_bytes, n_in, input_logs, exps, nullptr, 0, out_slots, n_out, accumulate,
+	                     (hipStream_t)stream);
+}
+
+extern "C" int pph_muladd_bilinear_coef(const void* in_slots, size_t slot_bytes, unsigned n_in,
+                                        const uint32_t* coef_log2, unsigned coef_stride, void* out_slots,
+                                        unsigned n_out, int accumulate, void* stream) {
+	if(n_out && !coef_log2) return PPH_E_INVALID;
+	return bilinear_call(in_slots, slot_bytes, n_in, nullptr, nullptr, coef_log2, coef_stride, out_slots, n_out,
+	                     accumulate, (hipStream_t)stream);
 }

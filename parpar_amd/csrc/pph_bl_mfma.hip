@@ -73,12 +73,16 @@ __device__ __forceinline__ T blm_uniform(const T& x) {
 }
 
 // Coefficient planes of one job's output block: outputs r0 .. r0 + n_rows
-// (exps already offset by r0), rows padded to r_pad with zeros.
+// (exps, or the explicit table, already offset by r0), rows padded to r_pad
+// with zeros.  Coefficients are PAR2's, 2^(in_logs[i] * exps[r] mod 65535),
+// or, with `coef`, explicit: coef[i * coef_stride + r] = 2 log2(c), 0xFFFFFFFF
+// for c = 0 (the encoding of pph_muladd_plain's table).
 struct BlmCoefJob {
 	uint32_t* cb;  // [t][kw_count][r_pad]
 	const uint16_t* in_logs;
 	const uint16_t* exps;
-	uint32_t kw_count, r_pad, n_in, n_rows;
+	const uint32_t* coef;
+	uint32_t kw_count, r_pad, n_in, n_rows, coef_stride, pad;
 };
 
 // Coefficient planes: job `one`, or job blockIdx.y of `table`.  Thread: one
@@ -94,12 +98,18 @@ __global__ __launch_bounds__(256) void k_blm_coef(BlmCoefJob one, const BlmCoefJ
 #pragma unroll
 		for(int b = 0; b < 16; b++) s[b] = 0;
 		if(r < J.n_rows) {
-			const uint32_t er = J.exps[r];
+			const uint32_t er = J.coef ? 0u : J.exps[r];
 			for(unsigned k = 0; k < 32; k++) {
 				const size_t i = kw * 32 + k;
 				if(i >= J.n_in) break;
-				uint32_t l = mod65535((uint32_t)J.in_logs[i] * er);
-				const uint32_t c = exp_tab[l == kOrder ? 0u : l];
+				uint32_t c;
+				if(J.coef) {
+					const uint32_t l2 = J.coef[i * J.coef_stride + r];
+					c = l2 == kSkipCoef ? 0u : exp_tab[l2 >> 1];
+				} else {
+					const uint32_t l = mod65535((uint32_t)J.in_logs[i] * er);
+					c = exp_tab[l == kOrder ? 0u : l];
+				}
 				const unsigned p = blm_pos_c(k);
 #pragma unroll
 				for(int b = 0; b < 16; b++) s[b] |= ((c >> b) & 1u) << p;
@@ -716,7 +726,7 @@ dim3 blm_combine_grid(size_t rows, size_t wbk, unsigned pieces) {
 
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const DeviceTables& tab, hipStream_t s, size_t cap) {
+                const DeviceTables& tab, hipStream_t s, size_t cap, const uint32_t* coef, unsigned coef_stride) {
 	if(!n_out) return PPH_OK;
 	const BlmShape sh = blm_shape(slot_bytes, n_in, n_out, cap);
 	if(!ws || sh.total == SIZE_MAX || ws_bytes < sh.total) return PPH_E_INVALID;
@@ -727,7 +737,16 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 	for(size_t r0 = 0; r0 < n_out; r0 += sh.r_blk) {
 		const unsigned rows = (unsigned)std::min<size_t>(sh.r_blk, n_out - r0);
 		const size_t r_pad = round_up(rows, sh.rt);
-		const BlmCoefJob cj{cb, input_logs, exps + r0, (uint32_t)sh.kw_count, (uint32_t)r_pad, n_in, rows};
+		const BlmCoefJob cj{cb,
+		                    input_logs,
+		                    coef ? nullptr : exps + r0,
+		                    coef ? coef + r0 : nullptr,
+		                    (uint32_t)sh.kw_count,
+		                    (uint32_t)r_pad,
+		                    n_in,
+		                    rows,
+		                    coef_stride,
+		                    0u};
 		hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(sh.kw_count * r_pad, tab)), dim3(256), 0, s, cj,
 		                   (const BlmCoefJob*)nullptr, tab.exp);
 		hipError_t e = hipGetLastError();
@@ -901,8 +920,8 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	for(unsigned k = 0; k < n; k++) {
 		sh[k] = blm_job_shape(jobs[k]);
 		cb[k] = reinterpret_cast<uint32_t*>(base + off);
-		b->coef.push_back(BlmCoefJob{cb[k], jobs[k].in_logs, jobs[k].exps, (uint32_t)sh[k].kw_count,
-		                             (uint32_t)sh[k].r_pad, jobs[k].n_in, jobs[k].n_out});
+		b->coef.push_back(BlmCoefJob{cb[k], jobs[k].in_logs, jobs[k].exps, nullptr, (uint32_t)sh[k].kw_count,
+		                             (uint32_t)sh[k].r_pad, jobs[k].n_in, jobs[k].n_out, 0u, 0u});
 		b->coef_threads = std::max(b->coef_threads, sh[k].kw_count * sh[k].r_pad);
 		off += sh[k].cb_bytes;
 	}

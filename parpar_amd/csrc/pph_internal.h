@@ -92,15 +92,21 @@ int autotune(struct ::pph_tuning* out);
 size_t bilinear_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_supported(size_t slot_bytes, unsigned n_in, unsigned n_out);
 int bilinear_reserve(size_t slot_bytes, unsigned n_in, unsigned n_out);
+// coef (optional): explicit coefficients as in pph_muladd_plain's table
 int bilinear_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                      const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                     const DeviceTables& tab, hipStream_t s);
+                     const DeviceTables& tab, hipStream_t s, const uint32_t* coef = nullptr, unsigned coef_stride = 0);
+// pph_muladd_bilinear with an optional explicit coefficient table
+int bilinear_call(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
+                  const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, void* out_slots, unsigned n_out,
+                  int accumulate, hipStream_t stream);
 // fp4 MFMA kernel of the bilinear form (pph_bl_mfma.hip).  cap > 0 chunks the
 // work so that the workspace stays within cap bytes (SIZE_MAX: it cannot).
 size_t blm_workspace_bytes(size_t slot_bytes, unsigned n_in, unsigned n_out, size_t cap = 0);
 int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const uint16_t* input_logs,
                 const uint16_t* exps, void* out_slots, unsigned n_out, bool accumulate, void* ws, size_t ws_bytes,
-                const DeviceTables& tab, hipStream_t s, size_t cap = 0);
+                const DeviceTables& tab, hipStream_t s, size_t cap = 0, const uint32_t* coef = nullptr,
+                unsigned coef_stride = 0);
 // Row tile of the bilinear GEMM for n_out outputs: the one that pads the
 // outputs least, a 64-row tile's rows weighed 1.42 (measured: its waves expand
 // ten masks per step where the others expand eight, and issue twice the

@@ -168,3 +168,60 @@ def test_bilinear_rejects_bad_shapes():
     assert lib().pph_muladd_bilinear(buf.ptr, 1000, 1, buf.ptr, buf.ptr, buf.ptr, 1, 0, None) == \
         parpar_amd.PPH_E_INVALID
     assert lib().pph_muladd_bilinear(buf.ptr, 1024, 1, None, None, buf.ptr, 1, 0, None) == parpar_amd.PPH_E_INVALID
+
+
+def gf_log_table():
+    """log2 over GF(2^16) with the PAR2 polynomial, from its definition."""
+    log = np.zeros(65536, np.uint32)
+    x = 1
+    for k in range(65535):
+        log[x] = k
+        x <<= 1
+        if x & 0x10000:
+            x ^= 0x1100B
+    return log
+
+
+@pytest.mark.parametrize("chunked", [False, True])
+def test_bilinear_explicit_coefficients(monkeypatch, chunked):
+    """pph_muladd_bilinear_coef (explicit coefficients, the decoding path) equals
+    the oracle's explicit-coefficient recovery, coefficients 0 and 1 included,
+    also with its output block and word chunk forced small."""
+    say("bilinear explicit coefficients", "chunked" if chunked else "")
+    n, n_out, S, stride = 700, 75, 10000, 80
+    rng = np.random.default_rng(11)
+    datas = np.stack([O.prng_bytes(74, 0, k, S) for k in range(n)])
+    coeffs = rng.integers(0, 65536, (n, n_out)).astype(np.uint16)
+    coeffs[rng.random((n, n_out)) < 0.1] = 0
+    coeffs[rng.random((n, n_out)) < 0.05] = 1
+    ref = O.recover(datas, [S] * n, S, coeffs=coeffs)
+    log = gf_log_table()
+    table = np.full((n, stride), 0xFFFFFFFF, np.uint32)
+    table[:, :n_out] = np.where(coeffs == 0, np.uint32(0xFFFFFFFF), 2 * log[coeffs])
+    _, slot = P.slot_layout(S, 64)
+    host = np.zeros((n, slot), np.uint8)
+    host[:, :S] = datas
+    src = P.DeviceBuffer(host.nbytes)
+    src.upload(host)
+    data = P.DeviceBuffer(host.nbytes)
+    parpar_amd.check(lib().pph_prep_slots_from(data.ptr, src.ptr, slot, n, None, S, 64, 0, None), "prep")
+    d_tab = P.DeviceBuffer(table.nbytes)
+    d_tab.upload(table)
+    out = P.DeviceBuffer(slot * n_out)
+    out.upload(np.full(slot * n_out, 0x5A, np.uint8))
+    lib().pph_bilinear_release()
+    if chunked:
+        monkeypatch.setenv("PPH_BILINEAR_ROWS", "32")
+        monkeypatch.setenv("PPH_BILINEAR_CHUNK_WORDS", "1024")
+    parpar_amd.check(lib().pph_muladd_bilinear_coef(data.ptr, slot, n, d_tab.ptr, stride, out.ptr, n_out, 0, None),
+                     "bilinear_coef")
+    valid = P.DeviceBuffer(4 * n_out)
+    parpar_amd.check(lib().pph_verify_slots(out.ptr, slot, n_out, S, 64, valid.ptr, None), "verify")
+    parpar_amd.check(lib().pph_synchronize())
+    lib().pph_bilinear_release()
+    got = out.download().reshape(n_out, slot)[:, :S]
+    assert (got == ref).all()
+    assert (valid.download(4 * n_out).view(np.int32) == 1).all()
+    # a table narrower than the outputs is refused
+    assert lib().pph_muladd_bilinear_coef(data.ptr, slot, n, d_tab.ptr, n_out - 1, out.ptr, n_out, 0, None) == \
+        parpar_amd.PPH_E_INVALID
