@@ -92,9 +92,8 @@ int pph_proc_create(int device, unsigned staging_areas, pph_proc** out);
  * gf16/gf16mul.cpp:215-409; 64 on AVX512+GFNI hosts); input_batch 0 picks the
  * default (256 inputs, fewer where a staging area would pass 512 MiB); form
  * is PPH_FORM_PLAIN (plain kernel), _BILINEAR (the bilinear
- * form for batches numbered by input; worth it with large input batches and
- * many recovery slices; explicit-coefficient batches still run the plain
- * kernel) or _AUTO (bilinear for input batches of at least 256 with at least
+ * form, for batches numbered by input and explicit-coefficient batches alike;
+ * worth it with large input batches and many recovery slices) or _AUTO (bilinear for input batches of at least 256 with at least
  * 64 recovery slices, plain otherwise; slots prepared by fill_input keep the
  * form chosen then, and a set_recovery_slices that would switch it returns
  * PPH_E_STATE).  setSliceSize must have been called. */
@@ -356,13 +355,17 @@ int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned cksum_size, 
 /* The form (PPH_FORM_PLAIN / _DENSE / _BILINEAR) chosen for a job, the plain
  * kernel's output grouping and the predicted mul-add time of the batch. */
 int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group, double* predicted_s);
-/* Run the batch on a stream: counter reset, per-job preparation, the fused
- * persistent mul-add (and dense / bilinear launches), per-job verification.
- * use_graph captures that sequence into a hipGraph on first use and replays
- * it.  A batch with bilinear jobs owns a device workspace for them. */
+/* Run the batch on a stream: preparation, the fused persistent mul-add of the
+ * plain jobs and the dense launches on a second stream of the batch (forked
+ * from `stream` and joined back), beside the bilinear jobs' preparation and
+ * launches on `stream`; then the verification of every job.  use_graph
+ * captures that work into a hipGraph on first use and replays it.  A batch
+ * with bilinear jobs owns a device workspace for them. */
 int pph_batch_run(pph_batch* b, int use_graph, void* stream);
 /* Time of the mul-add launches of the last eager (use_graph = 0) run, from HIP
- * events recorded on its stream around them; waits for that run. */
+ * events on its stream: from the end of the bilinear jobs' preparation to the
+ * end of every mul-add launch (the plain jobs' launch may start before it);
+ * waits for that run. */
 int pph_batch_fused_ms(pph_batch* b, float* ms);
 void pph_batch_destroy(pph_batch* b);
 

@@ -1,6 +1,7 @@
 // Batched execution: many independent PAR2 jobs (uneven input counts, slice
-// sizes and recovery counts) in one persistent fused mul-add launch, with the
-// per-job preparation and checksum verification around it captured once in a
+// sizes and recovery counts): the plain jobs in one persistent fused mul-add
+// launch, the bilinear jobs in cross-job waves, with the preparation and
+// checksum verification around them (one launch each) captured once in a
 // hipGraph and replayed.  The form of each job (plain, dense bit-matrix or
 // bilinear) and the plain kernel's output grouping are chosen per batch from
 // rates the autotuner measures on the device.
@@ -280,8 +281,17 @@ struct pph_batch {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	double predicted_s = 0;
-	hipEvent_t ev_a = nullptr, ev_b = nullptr;  // around the fused kernel (eager runs)
+	hipEvent_t ev_a = nullptr, ev_b = nullptr;  // around the mul-add launches (eager runs)
 	bool timing = false;
+	// second stream of the plain and dense jobs, forked from and joined back
+	// to the caller's stream
+	hipStream_t aux = nullptr;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	// one-launch preparation and verification tables (device): the bilinear
+	// jobs' preparation, the dense jobs', then every job's verification
+	CkJob* d_ck = nullptr;
+	unsigned ck_n[3] = {0, 0, 0}, ck_max[3] = {0, 0, 0};  // entries; slots in all, or the largest count
+	const CkJob* ck_table(unsigned k) const { return d_ck + (k > 0 ? ck_n[0] : 0) + (k > 1 ? ck_n[1] : 0); }
 };
 
 // Bilinear jobs' mul-adds: those of the plan in its cross-job launches, the
@@ -301,44 +311,58 @@ static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s) {
 	return hipSuccess;
 }
 
-// The batch's device work.
+// The batch's device work, on two streams: the plain and dense jobs (their
+// preparation, the dense launches, the persistent plain launch) on `aux`,
+// beside the bilinear jobs' preparation and launches on the caller's stream.
+// The plain launch is LDS-bound and the preparation memory-bound, so the two
+// share the CUs well; the bilinear GEMMs start once their inputs are prepared.
+// `aux` joins back before the verification of every job (one launch).
 static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
-	hipError_t e = hipSuccess;
+	hipError_t e = hipEventRecord(b->ev_fork, s);
+	if(e == hipSuccess) e = hipStreamWaitEvent(b->aux, b->ev_fork, 0);
+	if(e != hipSuccess) return e;
+	const hipStream_t a = b->aux;
 	if(b->n_items) {
-		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, b->d_counter, 0u);
+		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, a, b->d_counter, 0u);
 		e = hipGetLastError();
 	}
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
-		if(!jb.n_in || !jb.n_out) continue;
+		if(b->form[j] != PPH_FORM_PLAIN || !jb.n_in || !jb.n_out) continue;
 		e = launch_prep((uint8_t*)jb.in_slots, nullptr, jb.slot_bytes, jb.n_in, jb.in_lens, nullptr, jb.slice_size,
-		                ck_offset(jb.slice_size), b->ck, b->form[j] == PPH_FORM_PLAIN, *b->tab, s);
+		                ck_offset(jb.slice_size), b->ck, true, *b->tab, a);
 	}
-	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_a, s);
-	if(e == hipSuccess && b->n_items) {
-		dim3 grid(b->tab->cus > 0 ? b->tab->cus : 256), block(kPlainThreads);
-		if(b->group == 8)
-			hipLaunchKernelGGL(k_muladd_batch<8>, grid, block, 0, s, b->d_jobs, b->d_items, b->n_items, b->d_counter,
-			                   b->tab->exp);
-		else
-			hipLaunchKernelGGL(k_muladd_batch<16>, grid, block, 0, s, b->d_jobs, b->d_items, b->n_items, b->d_counter,
-			                   b->tab->exp);
-		e = hipGetLastError();
-	}
+	if(e == hipSuccess) e = launch_prep_data_jobs(b->ck_table(1), b->ck_n[1], b->ck_max[1], b->ck, 8, *b->tab, a);
+	// the dense launches (short) before the plain launch, which is the last
+	// on this stream and may still run when the first bilinear GEMM starts
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
 		if(b->form[j] != PPH_FORM_DENSE || !jb.n_out) continue;
 		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
-		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, s);
+		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, a);
 	}
-	if(e == hipSuccess) e = enqueue_bilinear(b, s);
+	if(e == hipSuccess && b->n_items) {
+		dim3 grid(b->tab->cus > 0 ? b->tab->cus : 256), block(kPlainThreads);
+		if(b->group == 8)
+			hipLaunchKernelGGL(k_muladd_batch<8>, grid, block, 0, a, b->d_jobs, b->d_items, b->n_items, b->d_counter,
+			                   b->tab->exp);
+		else
+			hipLaunchKernelGGL(k_muladd_batch<16>, grid, block, 0, a, b->d_jobs, b->d_items, b->n_items, b->d_counter,
+			                   b->tab->exp);
+		e = hipGetLastError();
+	}
+	// the join is recorded even after an error, so that a capture on `s` can end
+	const hipError_t ej = hipEventRecord(b->ev_join, a);
+	// two workgroups per CU and job: room beside them for the plain jobs' work
+	// (a 1024-thread workgroup with the 128 KiB table in LDS)
+	hipError_t eb = launch_prep_data_jobs(b->ck_table(0), b->ck_n[0], b->ck_max[0], b->ck, 2, *b->tab, s);
+	if(eb == hipSuccess && b->timing) eb = hipEventRecord(b->ev_a, s);
+	if(eb == hipSuccess) eb = enqueue_bilinear(b, s);
+	const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(s, b->ev_join, 0) : ej;
+	if(e == hipSuccess) e = eb;
+	if(e == hipSuccess) e = ew;
 	if(e == hipSuccess && b->timing) e = hipEventRecord(b->ev_b, s);
-	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
-		const pph_job& jb = b->jobs[j];
-		if(!jb.n_out || !jb.n_in) continue;
-		e = launch_verify((const uint8_t*)jb.out_slots, jb.slot_bytes, jb.n_out, jb.slice_size, ck_offset(jb.slice_size),
-		                  b->ck, jb.valid, *b->tab, s);
-	}
+	if(e == hipSuccess) e = launch_verify_jobs(b->ck_table(2), b->ck_n[2], b->ck_max[2], b->ck, *b->tab, s);
 	return e;
 }
 
@@ -352,8 +376,13 @@ extern "C" void pph_batch_destroy(pph_batch* b) {
 	if(b->d_jobs) (void)hipFree(b->d_jobs);
 	if(b->d_items) (void)hipFree(b->d_items);
 	if(b->d_counter) (void)hipFree(b->d_counter);
+	if(b->aux) (void)hipStreamSynchronize(b->aux);
 	if(b->ev_a) (void)hipEventDestroy(b->ev_a);
 	if(b->ev_b) (void)hipEventDestroy(b->ev_b);
+	if(b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+	if(b->ev_join) (void)hipEventDestroy(b->ev_join);
+	if(b->aux) (void)hipStreamDestroy(b->aux);
+	if(b->d_ck) (void)hipFree(b->d_ck);
 	blm_batch_destroy(b->bl_plan);
 	if(b->bl_ws) (void)hipFree(b->bl_ws);
 	delete b;
@@ -499,8 +528,42 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 			for(unsigned gp = 0; gp < grps; gp++) items.push_back(BatchItem{j, gp, c, 0});
 	}
 	b->n_items = (unsigned)items.size();
+	// preparation tables (largest slots first: the longest workgroups start
+	// first) and the verification table
+	std::vector<CkJob> ck_tab[3];
+	for(unsigned j = 0; j < n_jobs; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(!jb.n_in || !jb.n_out) continue;
+		const CkJob prep{(uint8_t*)jb.in_slots, jb.in_lens, nullptr, jb.slot_bytes, jb.slice_size,
+		                 ck_offset(jb.slice_size), jb.n_in, 0u};
+		const CkJob ver{(uint8_t*)jb.out_slots, nullptr, jb.valid, jb.slot_bytes, jb.slice_size,
+		                ck_offset(jb.slice_size), jb.n_out, 0u};
+		if(!ck_job_valid(prep, b->ck) || !ck_job_valid(ver, b->ck)) {
+			pph_batch_destroy(b);
+			return PPH_E_INVALID;
+		}
+		if(b->form[j] == PPH_FORM_BILINEAR) ck_tab[0].push_back(prep);
+		if(b->form[j] == PPH_FORM_DENSE) ck_tab[1].push_back(prep);
+		ck_tab[2].push_back(ver);
+	}
+	// ck_max: slots in all (preparation), the largest count (verification)
+	std::vector<CkJob> ck_all;
+	for(unsigned k = 0; k < 3; k++) {
+		if(k < 2)
+			std::stable_sort(ck_tab[k].begin(), ck_tab[k].end(),
+			                 [](const CkJob& x, const CkJob& y) { return x.slot_bytes > y.slot_bytes; });
+		b->ck_n[k] = (unsigned)ck_tab[k].size();
+		for(CkJob& c : ck_tab[k]) {
+			c.first = k < 2 ? b->ck_max[k] : 0u;
+			b->ck_max[k] = k < 2 ? b->ck_max[k] + c.count : std::max(b->ck_max[k], c.count);
+		}
+		ck_all.insert(ck_all.end(), ck_tab[k].begin(), ck_tab[k].end());
+	}
 	hipError_t e = hipSuccess;
-	if(n_jobs) e = hipMalloc(&b->d_jobs, n_jobs * sizeof(BatchJob));
+	if(!ck_all.empty()) e = hipMalloc(&b->d_ck, ck_all.size() * sizeof(CkJob));
+	if(e == hipSuccess && !ck_all.empty())
+		e = hipMemcpy(b->d_ck, ck_all.data(), ck_all.size() * sizeof(CkJob), hipMemcpyHostToDevice);
+	if(e == hipSuccess && n_jobs) e = hipMalloc(&b->d_jobs, n_jobs * sizeof(BatchJob));
 	if(e == hipSuccess && b->n_items) e = hipMalloc(&b->d_items, items.size() * sizeof(BatchItem));
 	if(e == hipSuccess) e = hipMalloc(&b->d_counter, 256);
 	if(e == hipSuccess && n_jobs) e = hipMemcpy(b->d_jobs, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice);
@@ -509,6 +572,9 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 	if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
 	if(e == hipSuccess) e = hipEventCreate(&b->ev_a);
 	if(e == hipSuccess) e = hipEventCreate(&b->ev_b);
+	if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
+	if(e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+	if(e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
 	if(e != hipSuccess) {
 		pph_batch_destroy(b);
 		return e == hipErrorOutOfMemory ? PPH_E_NOMEM : set_error(e);

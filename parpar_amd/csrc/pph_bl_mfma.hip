@@ -54,24 +54,6 @@ __device__ __forceinline__ uint4 blm_nibbles_c(uint32_t m) {
 	return make_uint4(m & 0x44444444u, m & 0x22222222u, m & 0x11111111u, (m >> 3) & 0x11111111u);
 }
 
-// A descriptor read from a table, made wave-uniform (scalar registers): the
-// tables are read with vector loads (the kernels also store to global memory),
-// which the compiler would otherwise treat as per-lane values.
-// (Word-wise bit casts: a byte-wise copy through memcpy made the compiler keep
-// the fields as separate bytes in scalar registers and reassemble them on
-// every use, ~100 instructions per GEMM item.)
-template <class T>
-__device__ __forceinline__ T blm_uniform(const T& x) {
-	static_assert(sizeof(T) % 4 == 0, "descriptor words");
-	struct Words {
-		uint32_t w[sizeof(T) / 4];
-	};
-	Words w = __builtin_bit_cast(Words, x);
-#pragma unroll
-	for(unsigned k = 0; k < sizeof(T) / 4; k++) w.w[k] = __builtin_amdgcn_readfirstlane(w.w[k]);
-	return __builtin_bit_cast(T, w);
-}
-
 // Coefficient planes of one job's output block: outputs r0 .. r0 + n_rows
 // (exps, or the explicit table, already offset by r0), rows padded to r_pad
 // with zeros.  Coefficients are PAR2's, 2^(in_logs[i] * exps[r] mod 65535),
@@ -89,7 +71,7 @@ struct BlmCoefJob {
 // output row r, one 32-input group kw; r varies fastest.
 __global__ __launch_bounds__(256) void k_blm_coef(BlmCoefJob one, const BlmCoefJob* __restrict__ table,
                                                   const uint16_t* __restrict__ exp_tab) {
-	const BlmCoefJob J = table ? blm_uniform(table[blockIdx.y]) : one;
+	const BlmCoefJob J = table ? wave_uniform(table[blockIdx.y]) : one;
 	const size_t total = (size_t)J.kw_count * J.r_pad;
 	for(size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
 		const size_t kw = g / J.r_pad;
@@ -156,7 +138,7 @@ struct BlmWordsPiece {
 // order), so after the transpose row b < 16 is bit slice b of the first word
 // and row 16 + b that of the second.
 __global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmWordsPiece* __restrict__ table) {
-	const BlmWordsPiece P = table ? blm_uniform(table[blockIdx.y]) : one;
+	const BlmWordsPiece P = table ? wave_uniform(table[blockIdx.y]) : one;
 	const unsigned per_kw = P.wc / 512;
 	const size_t units = (size_t)P.kw_count * per_kw;
 	const size_t plane = (size_t)P.kw_count * P.wc;
@@ -169,12 +151,12 @@ __global__ __launch_bounds__(256) void k_blm_words(BlmWordsPiece one, const BlmW
 			// The slot pointer and the group's input count are made wave-uniform
 			// explicitly: a descriptor the compiler takes for per-lane is issued in
 			// a loop over lanes (one per load).
-			const uint8_t* in = blm_uniform(P.in + 2 * (size_t)w0 + (size_t)kw * 32 * P.slot_bytes);
+			const uint8_t* in = wave_uniform(P.in + 2 * (size_t)w0 + (size_t)kw * 32 * P.slot_bytes);
 			const unsigned n = __builtin_amdgcn_readfirstlane(P.n_in > kw * 32 ? P.n_in - kw * 32 : 0);
 #pragma unroll
 			for(unsigned k = 0; k < 32; k++) {
 				// past the last input: an empty resource, whose loads return 0
-				const uint8_t* src = blm_uniform(in + k * P.slot_bytes);
+				const uint8_t* src = wave_uniform(in + k * P.slot_bytes);
 				a[4 * (k % 8) + k / 8] = __builtin_amdgcn_raw_buffer_load_b32(
 				    __builtin_amdgcn_make_buffer_rsrc((void*)src, 0, k < n ? 2048 : 0, 0x00020000), (int)(4 * tid), 0, 0);
 			}
@@ -276,7 +258,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 
 	// Item descriptors.  A workgroup's items only move forward through the
 	// table, so the piece is found by stepping from the last one.
-	BlmPiece P = table ? blm_uniform(table[0]) : one;
+	BlmPiece P = table ? wave_uniform(table[0]) : one;
 	unsigned pc = 0;
 	struct Src {
 		__amdgpu_buffer_rsrc_t a, b;
@@ -288,7 +270,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	auto locate = [&](unsigned j) {
 		const unsigned it = b + j * G;
 		if(table)
-			while(it >= P.item0 + P.items) P = blm_uniform(table[++pc]);
+			while(it >= P.item0 + P.items) P = wave_uniform(table[++pc]);
 		const unsigned li = it - P.item0;
 		unsigned tile, inner;
 		const unsigned t = blm_divmod(li, P.w_tiles * P.r_tiles, P.m_tiles, tile);
@@ -426,7 +408,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	};
 	Dst pd{P.par, 0u, 0, 0u};
 	auto store_prev = [&]() {
-		const Dst d = blm_uniform(pd);
+		const Dst d = wave_uniform(pd);
 		const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)d.base, 0, (int)d.bytes, kRsrcWord3);
 #pragma unroll
 		for(unsigned i = 0; i < NA; i++)
@@ -586,7 +568,7 @@ struct BlmCombinePiece {
 // in row 2i.  Each thread stores its 64 contiguous bytes as four 16-byte
 // stores (slots are 1 KiB multiples, chunks 512-word multiples).
 __global__ __launch_bounds__(256) void k_blm_combine(BlmCombinePiece one, const BlmCombinePiece* __restrict__ table) {
-	const BlmCombinePiece P = table ? blm_uniform(table[blockIdx.z]) : one;
+	const BlmCombinePiece P = table ? wave_uniform(table[blockIdx.z]) : one;
 	const unsigned r = blockIdx.x * 32 + (threadIdx.x & 31);
 	if(r >= P.n_rows) return;
 	const size_t plane = (size_t)P.plane_wbk * P.r_pad;
@@ -955,6 +937,11 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 		w.comb0 = (unsigned)comb.size();
 		for(auto& g : gemm) g.clear();
 		const size_t half = b->overlap ? (b->waves.size() % 2) * area : 0;
+		// Overlapped form: the first two waves fill a quarter and a half of their
+		// area (at least one piece each), so that the word planes of the first
+		// wave, which nothing runs beside, are short, and those of the second
+		// still fit under the first wave's GEMMs.
+		const size_t fill = !b->overlap || b->waves.size() > 1 ? area : b->waves.empty() ? area / 4 : area / 2;
 		size_t used = 0;
 		for(; p < pieces.size(); p++) {
 			const Piece& pc = pieces[p];
@@ -962,7 +949,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			const size_t wc = round_up(pc.wc, s.wt);  // plane row: pc.wc words, padded to the word tile
 			const size_t wb_bytes = round_up(kBl * s.kw_count * wc * 4, 256);
 			const size_t need = blm_piece_bytes(s, pc.wc);
-			if(used + need > area) {
+			if(used + need > (used ? fill : area)) {
 				if(!used) return PPH_E_INVALID;  // a piece larger than the area
 				break;
 			}

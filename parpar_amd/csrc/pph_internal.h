@@ -56,6 +56,23 @@ hipError_t launch_prep(uint8_t* slots, const uint8_t* src, size_t slot_bytes, un
                        bool to_log, const DeviceTables& t, hipStream_t s);
 hipError_t launch_verify(const uint8_t* slots, size_t slot_bytes, unsigned count, size_t slice_size,
                          size_t ck_off, unsigned ck, int* valid, const DeviceTables& t, hipStream_t s);
+// Slots [0, count) of one slot array in a launch over many (device-resident
+// tables): data-form preparation (lens, optional: bytes of each input; first:
+// number of slot 0 among the table's slots) or verification (valid: one flag
+// per slot).
+struct CkJob {
+	uint8_t* slots;
+	const uint32_t* lens;
+	int* valid;
+	uint64_t slot_bytes, slice_size, ck_off;
+	uint32_t count, first;
+};
+// Host-side check of one entry against the launchers' shape rules.
+bool ck_job_valid(const CkJob& j, unsigned ck);
+hipError_t launch_prep_data_jobs(const CkJob* table, unsigned n_jobs, unsigned total, unsigned ck, unsigned per_cu,
+                                 const DeviceTables& t, hipStream_t s);
+hipError_t launch_verify_jobs(const CkJob* table, unsigned n_jobs, unsigned max_count, unsigned ck,
+                              const DeviceTables& t, hipStream_t s);
 hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n_in,
                                const uint16_t* in_logs, const uint16_t* exps,
                                const uint32_t* coef, unsigned coef_stride,
@@ -179,6 +196,24 @@ struct Md5Span {
 	uint32_t pad;
 };
 constexpr uint32_t kMd5SpanFirst = 1, kMd5SpanFinal = 2;
+
+// A descriptor read from a table, made wave-uniform (scalar registers): the
+// tables are read with vector loads (the kernels also store to global memory),
+// which the compiler would otherwise treat as per-lane values.
+// (Word-wise bit casts: a byte-wise copy through memcpy made the compiler keep
+// the fields as separate bytes in scalar registers and reassemble them on
+// every use, ~100 instructions per GEMM item.)
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T& x) {
+	static_assert(sizeof(T) % 4 == 0, "descriptor words");
+	struct Words {
+		uint32_t w[sizeof(T) / 4];
+	};
+	Words w = __builtin_bit_cast(Words, x);
+#pragma unroll
+	for(unsigned k = 0; k < sizeof(T) / 4; k++) w.w[k] = __builtin_amdgcn_readfirstlane(w.w[k]);
+	return __builtin_bit_cast(T, w);
+}
 
 inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 inline size_t ck_offset(size_t slice_size) { return round_up(slice_size, 16); }

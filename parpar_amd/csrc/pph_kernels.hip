@@ -181,6 +181,54 @@ __device__ void ck_rows(const uint8_t* src, uint8_t* slot, size_t nrows, size_t 
 	if(r0 >= nb) run_end = (size_t)-1;  // run contributes nothing
 }
 
+// The rows of ck_rows for in-place data-form preparation (MODE 1, src ==
+// slot), in three runs per thread: rows wholly inside the input (read eight
+// at a time, no mask and no store: the common case, with few registers live,
+// so that few workgroups per CU still keep enough loads in flight), the row
+// holding the input's end (masked and stored), and the rows past it (zeros
+// stored without a read; Horner steps of zero while in the checksum).
+template <unsigned CK, unsigned T>
+__device__ void ck_rows_inplace(uint8_t* slot, size_t nrows, size_t nb, size_t len, uint32_t* h, size_t& run_end) {
+	using G = CkGeom<CK, T>;
+	constexpr unsigned U = 8;
+	const unsigned t = threadIdx.x % G::TPR;
+	const unsigned g = threadIdx.x / G::TPR;
+	const size_t rpg = (nrows + G::NG - 1) / G::NG;
+	const size_t r0 = (size_t)g * rpg;
+	const size_t r1 = r0 + rpg < nrows ? r0 + rpg : nrows;
+	const auto clamp_row = [&](size_t x) { return x < r0 ? r0 : (x > r1 ? r1 : x); };
+	const size_t rf = clamp_row(len / CK), rz = clamp_row((len + CK - 1) / CK);  // rows < rz < nb + 1
+	for(unsigned j = 0; j < G::DW; j++) h[j] = 0;
+	uint8_t* p = slot + (size_t)t * G::P;
+	size_t r = r0;
+	for(; r + U <= rf; r += U) {
+		Piece<G::DW> v[U];
+#pragma unroll
+		for(unsigned u = 0; u < U; u++) v[u] = load_piece<G::DW>(p + (r + u) * CK);
+#pragma unroll
+		for(unsigned u = 0; u < U; u++)
+			for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]) ^ v[u].d[j];
+	}
+	for(; r < rz; r++) {
+		Piece<G::DW> v = load_piece<G::DW>(p + r * CK);
+		const size_t off = r * CK + (size_t)t * G::P;
+		if(off + G::P > len) {
+			for(unsigned j = 0; j < G::DW; j++) v.d[j] &= keep_mask(off + 4 * j, len);
+			store_piece<G::DW>(p + r * CK, v);
+		}
+		for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]) ^ v.d[j];
+	}
+	Piece<G::DW> zero;
+	for(unsigned j = 0; j < G::DW; j++) zero.d[j] = 0;
+	for(; r < r1; r++) {
+		store_piece<G::DW>(p + r * CK, zero);
+		if(r < nb)
+			for(unsigned j = 0; j < G::DW; j++) h[j] = mul2x2(h[j]);
+	}
+	run_end = r1 < nb ? r1 : nb;
+	if(r0 >= nb) run_end = (size_t)-1;  // run contributes nothing
+}
+
 // Scale this thread's partial to its final position and reduce across groups.
 // Returns (in threads < CK/2) the checksum lane `threadIdx.x`.
 template <unsigned CK, unsigned T>
@@ -250,40 +298,82 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep(uint8_t* slots, const uin
 	}
 }
 
-// Data-form preparation (no log table): 256-thread workgroups, several per CU,
-// persistent over slots — the form the dense kernels read.
+// Data-form preparation (no log table) of one slot: zero padding past `len`,
+// gf16_cksum into the checksum block.
+// INPLACE: src == slot, known at compile time.
+template <unsigned CK, bool INPLACE>
+__device__ __forceinline__ void prep_data_slot(uint8_t* slot, const uint8_t* src, size_t slot_bytes, size_t len,
+                                               size_t slice_size, size_t ck_off, const uint16_t* __restrict__ lg,
+                                               const uint16_t* __restrict__ ex) {
+	const size_t nrows = slot_bytes / CK;
+	const size_t nb = (slice_size + CK - 1) / CK;
+	if(len > slice_size) len = slice_size;
+	uint32_t h[CkGeom<CK, kCkThreads>::DW];
+	size_t run_end;
+	if constexpr(INPLACE)
+		ck_rows_inplace<CK, kCkThreads>(slot, nrows, nb, len, h, run_end);
+	else
+		ck_rows<CK, kCkThreads, 1>(src, slot, nrows, nb, len, h, nullptr, run_end);
+	// row rewrites complete before the checksum block is stored over them: all
+	// of them come from this workgroup (one CU, one L2), so a workgroup barrier
+	// orders them; a device-scope fence would write back the whole L2 per slot
+	__syncthreads();
+	uint32_t lane = ck_reduce<CK, kCkThreads>(h, nb, run_end, lg, ex);
+	if(threadIdx.x < CK / 2) *reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = (uint16_t)lane;
+}
+
+// 256-thread workgroups, several per CU, persistent over slots — the form the
+// dense kernels read.
 template <unsigned CK>
 __global__ __launch_bounds__(kCkThreads) void k_prep_data(uint8_t* slots, const uint8_t* src_slots, size_t slot_bytes,
                                                           unsigned count, const uint32_t* __restrict__ lens,
                                                           const uint32_t* __restrict__ slot_index, size_t slice_size,
                                                           size_t ck_off, const uint16_t* __restrict__ lg,
                                                           const uint16_t* __restrict__ ex) {
-	const size_t nrows = slot_bytes / CK;
-	const size_t nb = (slice_size + CK - 1) / CK;
 	for(unsigned k = blockIdx.x; k < count; k += gridDim.x) {
 		const size_t sidx = slot_index ? slot_index[k] : k;
 		uint8_t* slot = slots + sidx * slot_bytes;
 		const uint8_t* src = src_slots ? src_slots + sidx * slot_bytes : slot;
-		size_t len = lens ? lens[k] : slice_size;
-		if(len > slice_size) len = slice_size;
-		uint32_t h[CkGeom<CK, kCkThreads>::DW];
-		size_t run_end;
-		ck_rows<CK, kCkThreads, 1>(src, slot, nrows, nb, len, h, nullptr, run_end);
-		// row rewrites complete before the checksum block is stored over them: all
-		// of them come from this workgroup (one CU, one L2), so a workgroup barrier
-		// orders them; a device-scope fence would write back the whole L2 per slot
-		__syncthreads();
-		uint32_t lane = ck_reduce<CK, kCkThreads>(h, nb, run_end, lg, ex);
-		if(threadIdx.x < CK / 2) *reinterpret_cast<uint16_t*>(slot + ck_off + 2 * threadIdx.x) = (uint16_t)lane;
+		prep_data_slot<CK, false>(slot, src, slot_bytes, lens ? lens[k] : slice_size, slice_size, ck_off, lg, ex);
 	}
 }
 
+// Many jobs: workgroup b prepares slots b, b + G, ... of the jobs' slots
+// numbered in table order (entry j's slot k is number table[j].first + k), so
+// a small grid can take its share of the CUs and leave the rest.
+template <unsigned CK>
+__global__ __launch_bounds__(kCkThreads) void k_prep_data_jobs(const CkJob* __restrict__ table, unsigned n_jobs,
+                                                               unsigned total, const uint16_t* __restrict__ lg,
+                                                               const uint16_t* __restrict__ ex) {
+	unsigned j = 0;
+	CkJob J = wave_uniform(table[0]);
+	for(unsigned n = blockIdx.x; n < total; n += gridDim.x) {
+		while(n >= J.first + J.count && j + 1 < n_jobs) J = wave_uniform(table[++j]);
+		const unsigned k = n - J.first;
+		uint8_t* slot = J.slots + (size_t)k * J.slot_bytes;
+		prep_data_slot<CK, true>(slot, slot, J.slot_bytes, J.lens ? J.lens[k] : J.slice_size, J.slice_size, J.ck_off, lg,
+		                         ex);
+	}
+}
+
+// One workgroup per slot; with `table`, slot blockIdx.x of job blockIdx.y
+// (workgroups past the job's slot count do nothing).
 template <unsigned CK>
 __global__ __launch_bounds__(kCkThreads) void k_verify(const uint8_t* __restrict__ slots, size_t slot_bytes,
                                                        size_t slice_size, size_t ck_off, int* __restrict__ valid,
                                                        const uint16_t* __restrict__ lg,
-                                                       const uint16_t* __restrict__ ex) {
+                                                       const uint16_t* __restrict__ ex,
+                                                       const CkJob* __restrict__ table) {
 	const unsigned k = blockIdx.x;
+	if(table) {
+		const CkJob J = wave_uniform(table[blockIdx.y]);
+		if(k >= J.count) return;
+		slots = J.slots;
+		slot_bytes = J.slot_bytes;
+		slice_size = J.slice_size;
+		ck_off = J.ck_off;
+		valid = J.valid;
+	}
 	uint8_t* slot = const_cast<uint8_t*>(slots) + (size_t)k * slot_bytes;
 	const size_t nb = (slice_size + CK - 1) / CK;
 	uint32_t h[CkGeom<CK, kCkThreads>::DW];
@@ -338,10 +428,49 @@ hipError_t launch_verify(const uint8_t* slots, size_t slot_bytes, unsigned count
 		return hipErrorInvalidValue;
 	dim3 grid(count), block(kCkThreads);
 	switch(ck) {
-		case 8: hipLaunchKernelGGL(k_verify<8>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
-		case 16: hipLaunchKernelGGL(k_verify<16>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
-		case 32: hipLaunchKernelGGL(k_verify<32>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
-		default: hipLaunchKernelGGL(k_verify<64>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp); break;
+		case 8: hipLaunchKernelGGL(k_verify<8>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp, (const CkJob*)nullptr); break;
+		case 16: hipLaunchKernelGGL(k_verify<16>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp, (const CkJob*)nullptr); break;
+		case 32: hipLaunchKernelGGL(k_verify<32>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp, (const CkJob*)nullptr); break;
+		default: hipLaunchKernelGGL(k_verify<64>, grid, block, 0, s, slots, slot_bytes, slice_size, ck_off, valid, t.log, t.exp, (const CkJob*)nullptr); break;
+	}
+	return hipGetLastError();
+}
+
+bool ck_job_valid(const CkJob& j, unsigned ck) {
+	return j.slots && j.slot_bytes % kSlotAlign == 0 && j.ck_off + ck <= j.slot_bytes && j.slice_size <= j.ck_off &&
+	       valid_ck(ck);
+}
+
+// Many jobs in one launch.  The entries were checked with ck_job_valid and
+// numbered (first) when the table was built; `total` slots in all.
+// Preparation: persistent, at most per_cu workgroups per CU, so that other
+// work can share the CUs.
+hipError_t launch_prep_data_jobs(const CkJob* table, unsigned n_jobs, unsigned total, unsigned ck, unsigned per_cu,
+                                 const DeviceTables& t, hipStream_t s) {
+	if(!n_jobs || !total) return hipSuccess;
+	if(!table || !valid_ck(ck) || !per_cu) return hipErrorInvalidValue;
+	const unsigned cus = (unsigned)(t.cus > 0 ? t.cus : 256);
+	dim3 grid(total < cus * per_cu ? total : cus * per_cu), block(kCkThreads);
+	switch(ck) {
+		case 8: hipLaunchKernelGGL(k_prep_data_jobs<8>, grid, block, 0, s, table, n_jobs, total, t.log, t.exp); break;
+		case 16: hipLaunchKernelGGL(k_prep_data_jobs<16>, grid, block, 0, s, table, n_jobs, total, t.log, t.exp); break;
+		case 32: hipLaunchKernelGGL(k_prep_data_jobs<32>, grid, block, 0, s, table, n_jobs, total, t.log, t.exp); break;
+		default: hipLaunchKernelGGL(k_prep_data_jobs<64>, grid, block, 0, s, table, n_jobs, total, t.log, t.exp); break;
+	}
+	return hipGetLastError();
+}
+
+// Verification: grid (largest slot count, jobs), one workgroup per slot.
+hipError_t launch_verify_jobs(const CkJob* table, unsigned n_jobs, unsigned max_count, unsigned ck,
+                              const DeviceTables& t, hipStream_t s) {
+	if(!n_jobs || !max_count) return hipSuccess;
+	if(!table || !valid_ck(ck) || n_jobs > 65535) return hipErrorInvalidValue;
+	dim3 grid(max_count, n_jobs), block(kCkThreads);
+	switch(ck) {
+		case 8: hipLaunchKernelGGL(k_verify<8>, grid, block, 0, s, nullptr, 0, 0, 0, nullptr, t.log, t.exp, table); break;
+		case 16: hipLaunchKernelGGL(k_verify<16>, grid, block, 0, s, nullptr, 0, 0, 0, nullptr, t.log, t.exp, table); break;
+		case 32: hipLaunchKernelGGL(k_verify<32>, grid, block, 0, s, nullptr, 0, 0, 0, nullptr, t.log, t.exp, table); break;
+		default: hipLaunchKernelGGL(k_verify<64>, grid, block, 0, s, nullptr, 0, 0, 0, nullptr, t.log, t.exp, table); break;
 	}
 	return hipGetLastError();
 }
