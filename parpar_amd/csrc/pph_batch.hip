@@ -313,10 +313,9 @@ static hipError_t enqueue_bilinear(pph_batch* b, hipStream_t s) {
 
 // The batch's device work, on two streams: the plain and dense jobs (their
 // preparation, the dense launches, the persistent plain launch) on `aux`,
-// beside the bilinear jobs' preparation and launches on the caller's stream.
-// The plain launch is LDS-bound and the preparation memory-bound, so the two
-// share the CUs well; the bilinear GEMMs start once their inputs are prepared.
-// `aux` joins back before the verification of every job (one launch).
+// beside the bilinear jobs' preparation and launches on the caller's stream,
+// so that neither waits for the other.  `aux` joins back before the
+// verification of every job (one launch).
 static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 	hipError_t e = hipEventRecord(b->ev_fork, s);
 	if(e == hipSuccess) e = hipStreamWaitEvent(b->aux, b->ev_fork, 0);
@@ -353,9 +352,13 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 	}
 	// the join is recorded even after an error, so that a capture on `s` can end
 	const hipError_t ej = hipEventRecord(b->ev_join, a);
-	// two workgroups per CU and job: room beside them for the plain jobs' work
-	// (a 1024-thread workgroup with the 128 KiB table in LDS)
-	hipError_t eb = launch_prep_data_jobs(b->ck_table(0), b->ck_n[0], b->ck_max[0], b->ck, 2, *b->tab, s);
+	// The bilinear jobs' preparation takes the whole device (eight workgroups
+	// per CU), so that their GEMMs start as early as possible; the plain jobs'
+	// work gets CUs as they free up (measured on the 64-job workload: the
+	// first GEMM at 18 ms and 489 ms per replay, against 36 ms and 500 ms with
+	// two preparation workgroups per CU, which leave room for the plain
+	// launches beside them but move the bytes at half the rate).
+	hipError_t eb = launch_prep_data_jobs(b->ck_table(0), b->ck_n[0], b->ck_max[0], b->ck, 8, *b->tab, s);
 	if(eb == hipSuccess && b->timing) eb = hipEventRecord(b->ev_a, s);
 	if(eb == hipSuccess) eb = enqueue_bilinear(b, s);
 	const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(s, b->ev_join, 0) : ej;
