@@ -237,7 +237,8 @@ __device__ __forceinline__ unsigned blm_divmod(unsigned n, unsigned d, unsigned 
 // exactly (n < 2^22 inputs), in [2, 4) where the encoding's unit is 2^-22: the
 // parity of n is bit 0 of the encoding.
 template <unsigned RT>
-__global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* __restrict__ table, unsigned items) {
+__global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* __restrict__ table, unsigned items,
+                                                  unsigned* __restrict__ counter) {
 	static_assert(RT == 256 || RT == 128 || RT == 64, "row tile");
 	constexpr unsigned WT = 65536 / RT;          // words per item
 	constexpr unsigned kA = 2 * WT / 256;        // word masks per thread and step
@@ -248,8 +249,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 	__shared__ __align__(16) uint32_t sA[kBufs][2][WT];
 	__shared__ __align__(16) uint32_t sB[kBufs][2][RT];
 	const unsigned G = gridDim.x, b = blockIdx.x;
-	const unsigned n_my = b < items ? (items - 1 - b) / G + 1 : 0;
-	if(!n_my) return;
+	if(b >= items) return;
 	const unsigned tid = threadIdx.x, lane = tid & 63;
 	const unsigned wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const unsigned wr = RT == 256 ? wave >> 1 : wave, wcol = RT == 256 ? wave & 1 : 0;
@@ -267,8 +267,7 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 		unsigned w0, r0, steps, lda, ldb;
 	};
 	constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit access, gfx9 family
-	auto locate = [&](unsigned j) {
-		const unsigned it = b + j * G;
+	auto locate = [&](unsigned it) {
 		if(table)
 			while(it >= P.item0 + P.items) P = wave_uniform(table[++pc]);
 		const unsigned li = it - P.item0;
@@ -358,8 +357,15 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 #pragma unroll
 		for(unsigned q = 0; q < kB; q++) m[kA + q] = __builtin_amdgcn_raw_buffer_load_b32(f.b, o.v[2 + q], sb, 0);
 	};
-	Src cur = locate(0);
-	Src nxt = locate(n_my > 1 ? 1 : 0);
+	// Items of this workgroup: b and b + G, then b + 2G, b + 3G, ... (counter
+	// null) or 2G plus the values `counter` hands out, one per item, so that a
+	// workgroup that starts late or shares its CU with other work takes fewer
+	// items.  The next index is taken by one thread at the start of an item and
+	// shared through LDS at its end (the atomic's latency lies under the item).
+	unsigned i_cur = b, i_nxt = b + G;
+	__shared__ unsigned s_next;
+	Src cur = locate(i_cur);
+	Src nxt = i_nxt < items ? locate(i_nxt) : cur;
 	Voff vc = voffsets(cur), vn = voffsets(nxt);
 
 	// Pipeline, step s (buffer and register-set indices static in the unrolled
@@ -482,7 +488,10 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 			if(k & 1) __syncthreads();
 		}
 	};
-	for(unsigned cj = 0; cj < n_my; cj++) {
+	for(;;) {
+		unsigned ticket = 0;
+		if(counter && i_nxt < items && tid == 0)
+			ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		// the first pass fetches steps 8 .. 11: of the next item when this one
 		// has eight steps
 		{
@@ -540,9 +549,20 @@ __global__ __launch_bounds__(256) void k_blm_gemm(BlmPiece one, const BlmPiece* 
 				po[i * NB / 2 + p] = sw[0] | sw[1];
 			}
 		pd = Dst{cur.pbase, cur.pbytes, sp, cur.ldb};
+		if(i_nxt >= items) break;
+		unsigned i_after = i_nxt + G;
+		if(counter) {
+			if(tid == 0) s_next = 2 * G + ticket;
+			__syncthreads();
+			i_after = __builtin_amdgcn_readfirstlane(s_next);
+		}
 		cur = nxt;
 		vc = vn;
-		nxt = locate(cj + 2 < n_my ? cj + 2 : n_my - 1);
+		i_cur = i_nxt;
+		i_nxt = i_after;
+		// past the last item: the current one again (its planes are fetched
+		// and never used)
+		if(i_nxt < items) nxt = locate(i_nxt);
 		vn = voffsets(nxt);
 	}
 	store_prev();
@@ -687,17 +707,23 @@ bool blm_planes_fit(size_t kw_count, size_t wc, size_t r_pad) {
 	return kw_count * wc * 4 < lim && kw_count * r_pad * 4 < lim && (wc / 32) * r_pad * 4 < lim;
 }
 
+// counter (optional): zeroed before the launch, items handed out dynamically
 hipError_t blm_launch_gemm(unsigned rt, const BlmPiece& one, const BlmPiece* table, unsigned items,
-                           const DeviceTables& tab, hipStream_t s) {
+                           const DeviceTables& tab, hipStream_t s, unsigned* counter = nullptr) {
 	if(!items) return hipSuccess;
 	const unsigned grid = std::min<unsigned>(items, (unsigned)(tab.cus > 0 ? tab.cus : 256));  // one per CU
 	if(rt == 64)
-		hipLaunchKernelGGL(k_blm_gemm<64>, dim3(grid), dim3(256), 0, s, one, table, items);
+		hipLaunchKernelGGL(k_blm_gemm<64>, dim3(grid), dim3(256), 0, s, one, table, items, counter);
 	else if(rt == 128)
-		hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, one, table, items);
+		hipLaunchKernelGGL(k_blm_gemm<128>, dim3(grid), dim3(256), 0, s, one, table, items, counter);
 	else
-		hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, one, table, items);
+		hipLaunchKernelGGL(k_blm_gemm<256>, dim3(grid), dim3(256), 0, s, one, table, items, counter);
 	return hipGetLastError();
+}
+
+// n counters to zero (a kernel: no memset node in a captured graph)
+__global__ void k_blm_zero(unsigned* p, unsigned n) {
+	for(unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
 }
 
 dim3 blm_combine_grid(size_t rows, size_t wbk, unsigned pieces) {
@@ -806,6 +832,7 @@ struct BlmBatch {
 	BlmWordsPiece* d_words = nullptr;
 	BlmPiece* d_gemm = nullptr;
 	BlmCombinePiece* d_comb = nullptr;
+	unsigned* d_counters = nullptr;  // item counters of the GEMM launches (3 per wave)
 	bool overlap = false;
 	hipStream_t side = nullptr;
 	hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_words[2] = {}, ev_gemm[2] = {};
@@ -885,6 +912,7 @@ void blm_batch_destroy(BlmBatch* b) {
 	if(b->d_words) (void)hipFree(b->d_words);
 	if(b->d_gemm) (void)hipFree(b->d_gemm);
 	if(b->d_comb) (void)hipFree(b->d_comb);
+	if(b->d_counters) (void)hipFree(b->d_counters);
 	for(hipEvent_t e : {b->ev_start, b->ev_end, b->ev_words[0], b->ev_words[1], b->ev_gemm[0], b->ev_gemm[1]})
 		if(e) (void)hipEventDestroy(e);
 	if(b->side) (void)hipStreamDestroy(b->side);
@@ -997,6 +1025,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	if(e == hipSuccess) e = upload(&b->d_words, words);
 	if(e == hipSuccess) e = upload(&b->d_gemm, gemm_all);
 	if(e == hipSuccess) e = upload(&b->d_comb, comb);
+	if(e == hipSuccess) e = hipMalloc((void**)&b->d_counters, 3 * b->waves.size() * sizeof(unsigned));
 	if(b->overlap) {
 		if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
 		for(hipEvent_t* ev : {&b->ev_start, &b->ev_end, &b->ev_words[0], &b->ev_words[1], &b->ev_gemm[0], &b->ev_gemm[1]})
@@ -1019,6 +1048,8 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 	const BlmCombinePiece none_m{};
 	hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(b->coef_threads, tab), (unsigned)b->coef.size()), dim3(256), 0, s,
 	                   none_c, b->d_coef, tab.exp);
+	const unsigned n_counters = 3 * (unsigned)b->waves.size();
+	hipLaunchKernelGGL(k_blm_zero, dim3((n_counters + 255) / 256), dim3(256), 0, s, b->d_counters, n_counters);
 	hipError_t e = hipGetLastError();
 	if(b->overlap) {
 		// Caller's stream: GEMM(k) after the word planes of wave k.  Side stream:
@@ -1043,7 +1074,8 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 			e = hipStreamWaitEvent(s, b->ev_words[k % 2], 0);
 			for(unsigned ti = 0; ti < 3 && e == hipSuccess; ti++)
 				if(w.gemm_n[ti])
-					e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
+					e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s,
+					                    b->d_counters + 3 * k + ti);
 			if(e == hipSuccess) e = hipEventRecord(b->ev_gemm[k % 2], s);
 			if(e == hipSuccess && k + 1 < b->waves.size()) {
 				e = words(b->waves[k + 1]);
@@ -1062,14 +1094,16 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 		if(e == hipSuccess) e = j2;
 		return e == hipSuccess ? PPH_OK : set_error(e);
 	}
-	for(const BlmBatch::Wave& w : b->waves) {
+	for(size_t k = 0; k < b->waves.size(); k++) {
+		const BlmBatch::Wave& w = b->waves[k];
 		if(e != hipSuccess) break;
 		hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(w.words_threads, tab), w.words_n), dim3(256), 0, s, none_w,
 		                   b->d_words + w.words0);
 		e = hipGetLastError();
 		for(unsigned ti = 0; ti < 3 && e == hipSuccess; ti++)
 			if(w.gemm_n[ti])
-				e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s);
+				e = blm_launch_gemm(256u >> ti, none_g, b->d_gemm + w.gemm0[ti], w.items[ti], tab, s,
+				                    b->d_counters + 3 * k + ti);
 		if(e == hipSuccess) {
 			hipLaunchKernelGGL(k_blm_combine, blm_combine_grid(w.comb_rows, w.comb_wbk, w.comb_n), dim3(256), 0, s,
 			                   none_m, b->d_comb + w.comb0);
