@@ -815,7 +815,7 @@ int blm_enqueue(const void* in_slots, size_t slot_bytes, unsigned n_in, const ui
 // wave's parity bits.  Those kernels are memory-bound and the GEMM is
 // MFMA-bound, and they fit on a CU beside a GEMM workgroup (which holds one
 // wave of ~380 registers per SIMD): the word planes run as about one
-// workgroup per CU for that reason.
+// workgroup per two CUs for that reason.
 struct BlmBatch {
 	std::vector<BlmCoefJob> coef;
 	struct Wave {
@@ -1057,11 +1057,16 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 		// GEMM(k - 1) and recombination(k - 1), both before it in this order),
 		// recombination(k) after GEMM(k).  The side stream joins back at the end.
 		const unsigned cus = (unsigned)(tab.cus > 0 ? tab.cus : 256);
-		// wave 0 runs alone (every CU); later waves run beside a GEMM (a
-		// workgroup per CU)
+		// wave 0 runs alone (every CU); later waves run beside a GEMM, as a
+		// workgroup per two CUs in all (the 64-job workload: 465-468 ms per
+		// replay, against 473-483 with one per CU, 478-481 with two and
+		// 473-478 with one per four; the GEMM hands out its items
+		// dynamically, so the CUs that host them take fewer items)
 		auto words = [&](const BlmBatch::Wave& w) {
 			const unsigned full = blm_blocks(w.words_threads, tab);
-			const unsigned gx = &w == &b->waves[0] ? full : std::min(full, std::max(1u, (cus + w.words_n - 1) / w.words_n));
+			const unsigned gx = &w == &b->waves[0]
+			                        ? full
+			                        : std::min(full, std::max(1u, (cus + 2 * w.words_n - 1) / (2 * w.words_n)));
 			hipLaunchKernelGGL(k_blm_words, dim3(gx, w.words_n), dim3(256), 0, b->side, none_w, b->d_words + w.words0);
 			return hipGetLastError();
 		};
