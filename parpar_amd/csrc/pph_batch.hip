@@ -484,7 +484,8 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 	for(unsigned j = 0; j < n_jobs; j++) {
 		if(b->form[j] != PPH_FORM_BILINEAR) continue;
 		const pph_job& jb = b->jobs[j];
-		const BlmBatchJob x{jb.in_slots, jb.slot_bytes, jb.n_in, jb.in_logs, jb.exps, jb.out_slots, jb.n_out};
+		const BlmBatchJob x{jb.in_slots, jb.slot_bytes, jb.n_in,  jb.in_logs,   jb.exps,
+		                    jb.out_slots, jb.n_out,      jb.in_lens, jb.slice_size};
 		if(blm_batch_eligible(x, budget)) {
 			bl.push_back(x);
 		} else {
@@ -501,7 +502,7 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		return PPH_E_NOMEM;
 	}
 	if(!bl.empty()) {
-		rc = blm_batch_create(bl.data(), (unsigned)bl.size(), b->bl_ws, b->bl_ws_bytes, &b->bl_plan);
+		rc = blm_batch_create(bl.data(), (unsigned)bl.size(), b->bl_ws, b->bl_ws_bytes, b->ck, &b->bl_plan);
 		if(rc) {
 			pph_batch_destroy(b);
 			return rc;
@@ -545,7 +546,8 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 			pph_batch_destroy(b);
 			return PPH_E_INVALID;
 		}
-		if(b->form[j] == PPH_FORM_BILINEAR) ck_tab[0].push_back(prep);
+		// the plan prepares its own jobs wave by wave
+		if(b->form[j] == PPH_FORM_BILINEAR && b->bl_alone[j]) ck_tab[0].push_back(prep);
 		if(b->form[j] == PPH_FORM_DENSE) ck_tab[1].push_back(prep);
 		ck_tab[2].push_back(ver);
 	}

@@ -833,6 +833,11 @@ struct BlmBatch {
 	BlmPiece* d_gemm = nullptr;
 	BlmCombinePiece* d_comb = nullptr;
 	unsigned* d_counters = nullptr;  // item counters of the GEMM launches (3 per wave)
+	// input preparation per wave (ck > 0): the jobs whose first piece is in
+	// the wave, entries prep0[k] .. prep0[k] + prep_n[k] of d_prep, slots in all
+	unsigned ck = 0;
+	CkJob* d_prep = nullptr;
+	std::vector<unsigned> prep0, prep_n, prep_slots;
 	bool overlap = false;
 	hipStream_t side = nullptr;
 	hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_words[2] = {}, ev_gemm[2] = {};
@@ -913,16 +918,18 @@ void blm_batch_destroy(BlmBatch* b) {
 	if(b->d_gemm) (void)hipFree(b->d_gemm);
 	if(b->d_comb) (void)hipFree(b->d_comb);
 	if(b->d_counters) (void)hipFree(b->d_counters);
+	if(b->d_prep) (void)hipFree(b->d_prep);
 	for(hipEvent_t e : {b->ev_start, b->ev_end, b->ev_words[0], b->ev_words[1], b->ev_gemm[0], b->ev_gemm[1]})
 		if(e) (void)hipEventDestroy(e);
 	if(b->side) (void)hipStreamDestroy(b->side);
 	delete b;
 }
 
-int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, BlmBatch** out) {
+int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, unsigned ck, BlmBatch** out) {
 	*out = nullptr;
 	std::unique_ptr<BlmBatch> b(new(std::nothrow) BlmBatch);
 	if(!b) return PPH_E_NOMEM;
+	b->ck = ck;
 	uint8_t* base = static_cast<uint8_t*>(ws);
 	size_t off = 0;
 	std::vector<BlmJobShape> sh(n);
@@ -954,6 +961,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	const char* ov = getenv("PPH_BATCH_OVERLAP");
 	b->overlap = pieces.size() >= 2 && area / 2 / 256 * 256 >= big && !(ov && ov[0] == '0');
 	if(b->overlap) area = area / 2 / 256 * 256;
+	std::vector<unsigned> first_wave(n, ~0u);
 	std::vector<BlmWordsPiece> words;
 	std::vector<BlmPiece> gemm[3];
 	std::vector<BlmCombinePiece> comb;
@@ -985,6 +993,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			uint32_t* par = reinterpret_cast<uint32_t*>(base + off + half + used + wb_bytes);
 			used += need;
 			const BlmBatchJob& j = jobs[pc.job];
+			if(first_wave[pc.job] == ~0u) first_wave[pc.job] = (unsigned)b->waves.size();
 			words.push_back(BlmWordsPiece{wbp, (const uint8_t*)j.in_slots + 2 * pc.w0, j.slot_bytes,
 			                              (uint32_t)s.kw_count, (uint32_t)wc, (uint32_t)pc.wc, j.n_in});
 			w.words_threads = std::max(w.words_threads, s.kw_count * wc / 2);
@@ -1014,6 +1023,33 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 		}
 		b->waves.push_back(w);
 	}
+	// preparation tables, one per wave (largest slots first), slots numbered
+	// within each table
+	std::vector<CkJob> prep;
+	if(ck) {
+		for(size_t k = 0; k < b->waves.size(); k++) {
+			std::vector<CkJob> t;
+			for(unsigned q = 0; q < n; q++) {
+				if(first_wave[q] != k) continue;
+				const BlmBatchJob& j = jobs[q];
+				const CkJob c{(uint8_t*)j.in_slots, j.in_lens, nullptr, j.slot_bytes, j.slice_size,
+				              ck_offset(j.slice_size), j.n_in, 0u};
+				if(!ck_job_valid(c, ck)) return PPH_E_INVALID;
+				t.push_back(c);
+			}
+			std::stable_sort(t.begin(), t.end(),
+			                 [](const CkJob& x, const CkJob& y) { return x.slot_bytes > y.slot_bytes; });
+			unsigned slots = 0;
+			for(CkJob& c : t) {
+				c.first = slots;
+				slots += c.count;
+			}
+			b->prep0.push_back((unsigned)prep.size());
+			b->prep_n.push_back((unsigned)t.size());
+			b->prep_slots.push_back(slots);
+			prep.insert(prep.end(), t.begin(), t.end());
+		}
+	}
 	auto upload = [](auto** d, const auto& v) -> hipError_t {
 		if(v.empty()) return hipSuccess;
 		const size_t bytes = v.size() * sizeof(v[0]);
@@ -1025,6 +1061,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	if(e == hipSuccess) e = upload(&b->d_words, words);
 	if(e == hipSuccess) e = upload(&b->d_gemm, gemm_all);
 	if(e == hipSuccess) e = upload(&b->d_comb, comb);
+	if(e == hipSuccess) e = upload(&b->d_prep, prep);
 	if(e == hipSuccess) e = hipMalloc((void**)&b->d_counters, 3 * b->waves.size() * sizeof(unsigned));
 	if(b->overlap) {
 		if(e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
@@ -1062,6 +1099,13 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 		// replay, against 473-483 with one per CU, 478-481 with two and
 		// 473-478 with one per four; the GEMM hands out its items
 		// dynamically, so the CUs that host them take fewer items)
+		// preparation of the jobs that start in wave k: on every CU before wave
+		// 0, beside a GEMM (one workgroup per CU) later
+		auto prep = [&](size_t k, unsigned per_cu) -> hipError_t {
+			if(!b->ck || !b->prep_n[k]) return hipSuccess;
+			return launch_prep_data_jobs(b->d_prep + b->prep0[k], b->prep_n[k], b->prep_slots[k], b->ck, per_cu, tab,
+			                             b->side);
+		};
 		auto words = [&](const BlmBatch::Wave& w) {
 			const unsigned full = blm_blocks(w.words_threads, tab);
 			const unsigned gx = &w == &b->waves[0]
@@ -1072,6 +1116,7 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 		};
 		if(e == hipSuccess) e = hipEventRecord(b->ev_start, s);
 		if(e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_start, 0);
+		if(e == hipSuccess) e = prep(0, 8);
 		if(e == hipSuccess) e = words(b->waves[0]);
 		if(e == hipSuccess) e = hipEventRecord(b->ev_words[0], b->side);
 		for(size_t k = 0; k < b->waves.size() && e == hipSuccess; k++) {
@@ -1083,7 +1128,8 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 					                    b->d_counters + 3 * k + ti);
 			if(e == hipSuccess) e = hipEventRecord(b->ev_gemm[k % 2], s);
 			if(e == hipSuccess && k + 1 < b->waves.size()) {
-				e = words(b->waves[k + 1]);
+				e = prep(k + 1, 1);
+				if(e == hipSuccess) e = words(b->waves[k + 1]);
 				if(e == hipSuccess) e = hipEventRecord(b->ev_words[(k + 1) % 2], b->side);
 			}
 			if(e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_gemm[k % 2], 0);
@@ -1102,6 +1148,10 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 	for(size_t k = 0; k < b->waves.size(); k++) {
 		const BlmBatch::Wave& w = b->waves[k];
 		if(e != hipSuccess) break;
+		if(b->ck && b->prep_n[k]) {
+			e = launch_prep_data_jobs(b->d_prep + b->prep0[k], b->prep_n[k], b->prep_slots[k], b->ck, 8, tab, s);
+			if(e != hipSuccess) break;
+		}
 		hipLaunchKernelGGL(k_blm_words, dim3(blm_blocks(w.words_threads, tab), w.words_n), dim3(256), 0, s, none_w,
 		                   b->d_words + w.words0);
 		e = hipGetLastError();
