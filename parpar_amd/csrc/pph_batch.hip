@@ -484,8 +484,8 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 	for(unsigned j = 0; j < n_jobs; j++) {
 		if(b->form[j] != PPH_FORM_BILINEAR) continue;
 		const pph_job& jb = b->jobs[j];
-		const BlmBatchJob x{jb.in_slots, jb.slot_bytes, jb.n_in,  jb.in_logs,   jb.exps,
-		                    jb.out_slots, jb.n_out,      jb.in_lens, jb.slice_size};
+		const BlmBatchJob x{jb.in_slots, jb.slot_bytes, jb.n_in,       jb.in_logs, jb.exps,
+		                    jb.out_slots, jb.n_out,     jb.in_lens, jb.slice_size, jb.valid};
 		if(blm_batch_eligible(x, budget)) {
 			bl.push_back(x);
 		} else {
@@ -546,10 +546,11 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 			pph_batch_destroy(b);
 			return PPH_E_INVALID;
 		}
-		// the plan prepares its own jobs wave by wave
-		if(b->form[j] == PPH_FORM_BILINEAR && b->bl_alone[j]) ck_tab[0].push_back(prep);
+		// the plan prepares and verifies its own jobs wave by wave
+		const bool planned = b->form[j] == PPH_FORM_BILINEAR && !b->bl_alone[j] && b->bl_plan;
+		if(b->form[j] == PPH_FORM_BILINEAR && !planned) ck_tab[0].push_back(prep);
 		if(b->form[j] == PPH_FORM_DENSE) ck_tab[1].push_back(prep);
-		ck_tab[2].push_back(ver);
+		if(!planned) ck_tab[2].push_back(ver);
 	}
 	// ck_max: slots in all (preparation), the largest count (verification)
 	std::vector<CkJob> ck_all;

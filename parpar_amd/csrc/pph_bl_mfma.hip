@@ -834,10 +834,12 @@ struct BlmBatch {
 	BlmCombinePiece* d_comb = nullptr;
 	unsigned* d_counters = nullptr;  // item counters of the GEMM launches (3 per wave)
 	// input preparation per wave (ck > 0): the jobs whose first piece is in
-	// the wave, entries prep0[k] .. prep0[k] + prep_n[k] of d_prep, slots in all
+	// the wave, entries prep0[k] .. prep0[k] + prep_n[k] of d_prep, slots in
+	// all; output verification per wave: the jobs whose last piece is in it,
+	// entries ver0[k] .. of d_prep, the largest output count
 	unsigned ck = 0;
 	CkJob* d_prep = nullptr;
-	std::vector<unsigned> prep0, prep_n, prep_slots;
+	std::vector<unsigned> prep0, prep_n, prep_slots, ver0, ver_n, ver_max;
 	bool overlap = false;
 	hipStream_t side = nullptr;
 	hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_words[2] = {}, ev_gemm[2] = {};
@@ -961,7 +963,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 	const char* ov = getenv("PPH_BATCH_OVERLAP");
 	b->overlap = pieces.size() >= 2 && area / 2 / 256 * 256 >= big && !(ov && ov[0] == '0');
 	if(b->overlap) area = area / 2 / 256 * 256;
-	std::vector<unsigned> first_wave(n, ~0u);
+	std::vector<unsigned> first_wave(n, ~0u), last_wave(n, 0u);
 	std::vector<BlmWordsPiece> words;
 	std::vector<BlmPiece> gemm[3];
 	std::vector<BlmCombinePiece> comb;
@@ -994,6 +996,7 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			used += need;
 			const BlmBatchJob& j = jobs[pc.job];
 			if(first_wave[pc.job] == ~0u) first_wave[pc.job] = (unsigned)b->waves.size();
+			last_wave[pc.job] = (unsigned)b->waves.size();
 			words.push_back(BlmWordsPiece{wbp, (const uint8_t*)j.in_slots + 2 * pc.w0, j.slot_bytes,
 			                              (uint32_t)s.kw_count, (uint32_t)wc, (uint32_t)pc.wc, j.n_in});
 			w.words_threads = std::max(w.words_threads, s.kw_count * wc / 2);
@@ -1049,6 +1052,22 @@ int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_by
 			b->prep_slots.push_back(slots);
 			prep.insert(prep.end(), t.begin(), t.end());
 		}
+		for(size_t k = 0; k < b->waves.size(); k++) {
+			unsigned cnt = 0, mx = 0;
+			b->ver0.push_back((unsigned)prep.size());
+			for(unsigned q = 0; q < n; q++) {
+				if(last_wave[q] != k || !jobs[q].n_out) continue;
+				const BlmBatchJob& j = jobs[q];
+				const CkJob c{(uint8_t*)j.out_slots, nullptr, j.valid, j.slot_bytes, j.slice_size, ck_offset(j.slice_size),
+				              j.n_out, 0u};
+				if(!j.valid || !ck_job_valid(c, ck)) return PPH_E_INVALID;
+				prep.push_back(c);
+				cnt++;
+				mx = std::max(mx, j.n_out);
+			}
+			b->ver_n.push_back(cnt);
+			b->ver_max.push_back(mx);
+		}
 	}
 	auto upload = [](auto** d, const auto& v) -> hipError_t {
 		if(v.empty()) return hipSuccess;
@@ -1085,6 +1104,11 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 	const BlmCombinePiece none_m{};
 	hipLaunchKernelGGL(k_blm_coef, dim3(blm_blocks(b->coef_threads, tab), (unsigned)b->coef.size()), dim3(256), 0, s,
 	                   none_c, b->d_coef, tab.exp);
+	// verification of the jobs that end in wave k (after its recombination)
+	auto verify = [&](size_t k, hipStream_t st) -> hipError_t {
+		if(!b->ck || !b->ver_n[k]) return hipSuccess;
+		return launch_verify_jobs(b->d_prep + b->ver0[k], b->ver_n[k], b->ver_max[k], b->ck, tab, st);
+	};
 	const unsigned n_counters = 3 * (unsigned)b->waves.size();
 	hipLaunchKernelGGL(k_blm_zero, dim3((n_counters + 255) / 256), dim3(256), 0, s, b->d_counters, n_counters);
 	hipError_t e = hipGetLastError();
@@ -1138,6 +1162,7 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 				                   b->side, none_m, b->d_comb + w.comb0);
 				e = hipGetLastError();
 			}
+			if(e == hipSuccess) e = verify(k, b->side);
 		}
 		// join even after an error, so that a capture on `s` can end
 		const hipError_t j = hipEventRecord(b->ev_end, b->side);
@@ -1164,6 +1189,7 @@ int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s)
 			                   none_m, b->d_comb + w.comb0);
 			e = hipGetLastError();
 		}
+		if(e == hipSuccess) e = verify(k, s);
 	}
 	return e == hipSuccess ? PPH_OK : set_error(e);
 }

@@ -172,12 +172,14 @@ struct BlmBatchJob {
 	unsigned n_out;
 	const uint32_t* in_lens;  // preparation: bytes of each input (null: slice_size)
 	size_t slice_size;
+	int* valid;  // verification: n_out checksum flags
 };
 struct BlmBatch;
 bool blm_batch_eligible(const BlmBatchJob& j, size_t ws_budget);
 size_t blm_batch_workspace(const BlmBatchJob* jobs, unsigned n, size_t ws_budget);
 // ck > 0: the batch also prepares the jobs' inputs (data form, checksum width
-// ck), each job before the word planes of its first wave
+// ck), each job before the word planes of its first wave, and verifies their
+// outputs' checksums after the recombination of its last wave
 int blm_batch_create(const BlmBatchJob* jobs, unsigned n, void* ws, size_t ws_bytes, unsigned ck, BlmBatch** out);
 int blm_batch_enqueue(const BlmBatch* b, const DeviceTables& tab, hipStream_t s);
 void blm_batch_destroy(BlmBatch* b);
