@@ -543,7 +543,8 @@ def main_batched(args):
     # Roofline of the bilinear jobs (all but the smallest; most of the mul-adds):
     # their fp4 MFMA work, 81 components x 2 ops per mul-add, over the fused
     # region's HIP-event time (which also holds the word-plane, recombination,
-    # plain and dense launches).  executed_mfma_ops adds the GEMM's padding
+    # the bilinear jobs' preparation and verification, and the plain and dense
+    # launches).  executed_mfma_ops adds the GEMM's padding
     # (inputs to 256, at least 512; outputs to the job's row tiles of 256, 128
     # and 64 rows, split as the library splits them).
     bl_ops = bl_exec = 0
@@ -571,7 +572,7 @@ def main_batched(args):
         "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": FP4_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tf / FP4_PEAK_TFLOPS, 4), "traffic": None,
                      "kernel": "k_blm_gemm over the bilinear jobs (fused region: + word planes, recombination, "
-                               "k_muladd_batch)",
+                               "the bilinear jobs' preparation and verification, k_muladd_batch)",
                      "kernel_ms": round(fused_ms, 3), "algorithmic_mfma_ops": bl_ops,
                      "executed_mfma_ops": bl_exec, "operand_type": "fp4 e2m1",
                      "hbm_equivalent": {"achieved": round(hbm_eq, 2), "unit": "GB/s",

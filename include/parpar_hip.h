@@ -363,9 +363,9 @@ int pph_batch_plan(const pph_batch* b, unsigned job, int* form, unsigned* group,
  * with bilinear jobs owns a device workspace for them. */
 int pph_batch_run(pph_batch* b, int use_graph, void* stream);
 /* Time of the mul-add launches of the last eager (use_graph = 0) run, from HIP
- * events on its stream: from the end of the bilinear jobs' preparation to the
- * end of every mul-add launch (the plain jobs' launch may start before it);
- * waits for that run. */
+ * events on its stream: from the start of the bilinear launches (which
+ * prepare and verify their jobs wave by wave) to the end of every mul-add
+ * launch (the plain jobs' work may start before it); waits for that run. */
 int pph_batch_fused_ms(pph_batch* b, float* ms);
 void pph_batch_destroy(pph_batch* b);
 
