@@ -73,11 +73,14 @@ hipError_t launch_prep_data_jobs(const CkJob* table, unsigned n_jobs, unsigned t
                                  const DeviceTables& t, hipStream_t s);
 hipError_t launch_verify_jobs(const CkJob* table, unsigned n_jobs, unsigned max_count, unsigned ck,
                               const DeviceTables& t, hipStream_t s);
+// split: persistent grid with the work split evenly over the CUs (items
+// shared by two workgroups combined with atomic XORs); false: one workgroup
+// per (column tile, output group) item, for launches that share the CUs
 hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n_in,
                                const uint16_t* in_logs, const uint16_t* exps,
                                const uint32_t* coef, unsigned coef_stride,
                                uint16_t* out, unsigned n_out, bool accumulate, unsigned group,
-                               const DeviceTables& t, hipStream_t s);
+                               const DeviceTables& t, hipStream_t s, bool split = true);
 hipError_t launch_muladd_dense(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
                                const uint16_t* exps, uint16_t* out, unsigned n_out, bool accumulate,
                                const DeviceTables& t, hipStream_t s);

@@ -477,6 +477,16 @@ hipError_t launch_verify_jobs(const CkJob* table, unsigned n_jobs, unsigned max_
 
 // ------------------------------------------------------------------ plain mul-add
 
+// Persistent, one workgroup per CU, over the units (item, input), item =
+// (column tile, output group) numbered column-major, split evenly between the
+// workgroups (a "stream-K" split): a workgroup takes a run of whole items
+// and, at its ends, parts of the items it shares with its neighbours, whose
+// sums it XORs into the output with atomics.  With one workgroup per item
+// (845 items for 1 MiB slices and 200 outputs in groups of 16) the last of
+// ~3.3 rounds of workgroups ran a quarter of the CUs: 77 % of the LDS
+// ceiling against ~95 % per workgroup.  The shares go to the workgroups in
+// XCD-major order (blockIdx round-robins over the 8 XCDs), so that the
+// items of one column tile, which read the same inputs, meet in one L2.
 template <int G, bool TABLE>
 __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
     const uint16_t* __restrict__ in, size_t slot_words, unsigned n_in, const uint16_t* __restrict__ in_logs,
@@ -487,8 +497,37 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
 	__shared__ uint16_t s_lin[kPlainThreads];
 	load_exp_table(s_exp, exp_tab);
 	__syncthreads();
+	const unsigned P = gridDim.x;
+	const unsigned sh = (P & 7u) == 0 ? (blockIdx.x & 7u) * (P >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+	const uint64_t total = (uint64_t)col_tiles * groups * n_in;
+	uint64_t u = total * sh / P;
+	const uint64_t u1 = total * (sh + 1) / P;
+	while(u < u1) {
+		const uint64_t item = u / n_in;
+		const unsigned i0 = (unsigned)(u - item * n_in);
+		const uint64_t left = u1 - u;
+		const unsigned i1 = left < (uint64_t)(n_in - i0) ? i0 + (unsigned)left : n_in;
+		const unsigned col = (unsigned)(item / groups);
+		const unsigned grp = (unsigned)(item - (uint64_t)col * groups);
+		plain_tile<G, TABLE>(s_exp, s_lin, in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate,
+		                     col, grp, i0, i1, i0 != 0 || i1 != n_in);
+		u += i1 - i0;
+	}
+}
 
-	// blocks of one column tile run on the same XCD (blockIdx round-robins over 8 XCDs)
+// One workgroup per item (the form concurrent launches share the CUs in, as
+// the serving engine runs them): blocks of one column tile run on the same
+// XCD (blockIdx round-robins over 8 XCDs).
+template <int G, bool TABLE>
+__global__ __launch_bounds__(kPlainThreads) void k_muladd_plain_items(
+    const uint16_t* __restrict__ in, size_t slot_words, unsigned n_in, const uint16_t* __restrict__ in_logs,
+    const uint16_t* __restrict__ exps, const uint32_t* __restrict__ coef, unsigned coef_stride,
+    uint16_t* __restrict__ out, unsigned n_out, int accumulate, const uint16_t* __restrict__ exp_tab,
+    unsigned col_tiles, unsigned groups) {
+	__shared__ uint16_t s_exp[65536];
+	__shared__ uint16_t s_lin[kPlainThreads];
+	load_exp_table(s_exp, exp_tab);
+	__syncthreads();
 	const unsigned total = col_tiles * groups;
 	unsigned L = blockIdx.x;
 	if((total & 7u) == 0) L = (blockIdx.x & 7u) * (total >> 3) + (blockIdx.x >> 3);
@@ -498,14 +537,57 @@ __global__ __launch_bounds__(kPlainThreads) void k_muladd_plain(
 	                     grp);
 }
 
+// Before an overwriting launch: the output tiles of the items that the split
+// shares between workgroups (those a share starts inside), zeroed for the
+// atomic XORs.  Block (s, r): the start of share s, row r of its group.
+template <int G>
+__global__ __launch_bounds__(kPlainThreads) void k_plain_zero_split(uint16_t* __restrict__ out, size_t slot_words,
+                                                                    unsigned n_in, unsigned n_out, unsigned col_tiles,
+                                                                    unsigned groups) {
+	const uint64_t total = (uint64_t)col_tiles * groups * n_in;
+	const uint64_t u = total * blockIdx.x / gridDim.x;
+	if(u >= total || u % n_in == 0) return;
+	const uint64_t item = u / n_in;
+	const unsigned col = (unsigned)(item / groups);
+	const unsigned row = (unsigned)(item - (uint64_t)col * groups) * G + blockIdx.y;
+	const size_t w0 = (size_t)col * (kPlainThreads * kPlainWordsPerLane) + (size_t)threadIdx.x * kPlainWordsPerLane;
+	if(row < n_out && w0 < slot_words)
+		*reinterpret_cast<uint4*>(out + (size_t)row * slot_words + w0) = make_uint4(0, 0, 0, 0);
+}
+
+// rows [0, n_out) of the output slots zeroed (an overwriting launch without inputs)
+__global__ void k_plain_zero_rows(uint16_t* __restrict__ out, size_t slot_words, unsigned n_out) {
+	const size_t n = (size_t)n_out * slot_words / 8;
+	for(size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+		reinterpret_cast<uint4*>(out)[k] = make_uint4(0, 0, 0, 0);
+}
+
 template <int G>
 static hipError_t launch_plain_g(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
                                  const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, uint16_t* out,
-                                 unsigned n_out, bool accumulate, const DeviceTables& t, hipStream_t s) {
+                                 unsigned n_out, bool accumulate, bool split, const DeviceTables& t, hipStream_t s) {
 	const unsigned tile_words = kPlainThreads * kPlainWordsPerLane;
 	const unsigned col_tiles = (unsigned)((slot_words + tile_words - 1) / tile_words);
 	const unsigned groups = (n_out + G - 1) / G;
-	dim3 grid(col_tiles * groups), block(kPlainThreads);
+	if(!split) {
+		dim3 grid(col_tiles * groups), block(kPlainThreads);
+		if(coef)
+			hipLaunchKernelGGL((k_muladd_plain_items<G, true>), grid, block, 0, s, in, slot_words, n_in, in_logs, exps,
+			                   coef, coef_stride, out, n_out, accumulate ? 1 : 0, t.exp, col_tiles, groups);
+		else
+			hipLaunchKernelGGL((k_muladd_plain_items<G, false>), grid, block, 0, s, in, slot_words, n_in, in_logs, exps,
+			                   coef, coef_stride, out, n_out, accumulate ? 1 : 0, t.exp, col_tiles, groups);
+		return hipGetLastError();
+	}
+	if(!n_in) {
+		if(!accumulate) hipLaunchKernelGGL(k_plain_zero_rows, dim3(1024), dim3(256), 0, s, out, slot_words, n_out);
+		return hipGetLastError();
+	}
+	const unsigned P = (unsigned)(t.cus > 0 ? t.cus : 256);  // one workgroup per CU
+	if(!accumulate)
+		hipLaunchKernelGGL((k_plain_zero_split<G>), dim3(P, G), dim3(kPlainThreads), 0, s, out, slot_words, n_in, n_out,
+		                   col_tiles, groups);
+	dim3 grid(P), block(kPlainThreads);
 	if(coef)
 		hipLaunchKernelGGL((k_muladd_plain<G, true>), grid, block, 0, s, in, slot_words, n_in, in_logs, exps, coef,
 		                   coef_stride, out, n_out, accumulate ? 1 : 0, t.exp, col_tiles, groups);
@@ -518,7 +600,7 @@ static hipError_t launch_plain_g(const uint16_t* in, size_t slot_words, unsigned
 hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n_in, const uint16_t* in_logs,
                                const uint16_t* exps, const uint32_t* coef, unsigned coef_stride, uint16_t* out,
                                unsigned n_out, bool accumulate, unsigned group, const DeviceTables& t,
-                               hipStream_t s) {
+                               hipStream_t s, bool split) {
 	if(!n_out || !slot_words) return hipSuccess;
 	if(slot_words % (kSlotAlign / 2)) return hipErrorInvalidValue;
 	if(!coef && (!in_logs || !exps) && n_in) return hipErrorInvalidValue;
@@ -532,11 +614,11 @@ hipError_t launch_muladd_plain(const uint16_t* in, size_t slot_words, unsigned n
 		if(coef_stride < padded) return hipErrorInvalidValue;
 	}
 	switch(group) {
-		case 1: return launch_plain_g<1>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
-		case 2: return launch_plain_g<2>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
-		case 4: return launch_plain_g<4>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
-		case 8: return launch_plain_g<8>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
-		case 16: return launch_plain_g<16>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, t, s);
+		case 1: return launch_plain_g<1>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, split, t, s);
+		case 2: return launch_plain_g<2>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, split, t, s);
+		case 4: return launch_plain_g<4>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, split, t, s);
+		case 8: return launch_plain_g<8>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, split, t, s);
+		case 16: return launch_plain_g<16>(in, slot_words, n_in, in_logs, exps, coef, coef_stride, out, n_out, accumulate, split, t, s);
 		default: return hipErrorInvalidValue;
 	}
 }

@@ -32,16 +32,22 @@ __device__ __forceinline__ void load_exp_table(uint16_t* s_exp, const uint16_t* 
 	for(unsigned k = threadIdx.x; k < 65536 * 2 / 16; k += kPlainThreads) dst[k] = src[k];
 }
 
-// One work item: column tile `col` (8192 words) x recovery slices [G*grp, G*grp+G).
-// s_lin: LDS for kPlainThreads input logs.  Every thread of the workgroup
-// calls it (it synchronises the workgroup).
+// One work item: column tile `col` (8192 words) x recovery slices [G*grp, G*grp+G),
+// over inputs [i0, i1).  partial: the item's other inputs are summed by other
+// workgroups, so the sums are XORed into `out` with atomics (which then holds
+// zeros or, accumulating, the previous output); otherwise they are stored
+// (accumulate: XORed with the previous output).  s_lin: LDS for
+// kPlainThreads input logs.  Every thread of the workgroup calls it (it
+// synchronises the workgroup).
 template <int G, bool TABLE>
 __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_lin, const uint16_t* __restrict__ in,
                                            size_t slot_words,
                                            unsigned n_in, const uint16_t* __restrict__ in_logs,
                                            const uint16_t* __restrict__ exps, const uint32_t* __restrict__ coef,
                                            unsigned coef_stride, uint16_t* __restrict__ out, unsigned n_out,
-                                           int accumulate, unsigned col, unsigned grp) {
+                                           int accumulate, unsigned col, unsigned grp, unsigned i0 = 0,
+                                           unsigned i1 = ~0u, bool partial = false) {
+	if(i1 > n_in) i1 = n_in;
 	const size_t w0 = (size_t)col * (kPlainThreads * kPlainWordsPerLane) + (size_t)threadIdx.x * kPlainWordsPerLane;
 	const bool active = w0 < slot_words;
 	const unsigned r0 = grp * G;
@@ -56,7 +62,7 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_li
 #pragma unroll
 	for(int r = 0; r < G; r++) {
 		acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
-		if(accumulate && active && r0 + r < n_out) {
+		if(accumulate && !partial && active && r0 + r < n_out) {
 			uint4 v = *reinterpret_cast<const uint4*>(out + (size_t)(r0 + r) * slot_words + w0);
 			acc[r][0] = v.x; acc[r][1] = v.y; acc[r][2] = v.z; acc[r][3] = v.w;
 		}
@@ -68,15 +74,15 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_li
 	// into a scalar register waited for every load in flight).
 	const uint4 kNone = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
 	uint4 v_next = kNone;
-	if(n_in && active) v_next = *reinterpret_cast<const uint4*>(in + w0);
-	for(unsigned i = 0; i < n_in; i++) {
+	if(i0 < i1 && active) v_next = *reinterpret_cast<const uint4*>(in + (size_t)i0 * slot_words + w0);
+	for(unsigned i = i0; i < i1; i++) {
 		const uint4 v = v_next;
-		if(i + 1 < n_in && active) v_next = *reinterpret_cast<const uint4*>(in + (size_t)(i + 1) * slot_words + w0);
+		if(i + 1 < i1 && active) v_next = *reinterpret_cast<const uint4*>(in + (size_t)(i + 1) * slot_words + w0);
 		if constexpr(!TABLE) {
-			if(i % kPlainThreads == 0) {
+			if((i - i0) % kPlainThreads == 0) {
 				__syncthreads();  // the previous chunk has been read by every wave
 				const unsigned k = i + threadIdx.x;
-				s_lin[threadIdx.x] = k < n_in ? in_logs[k] : 0;
+				s_lin[threadIdx.x] = k < i1 ? in_logs[k] : 0;
 				__syncthreads();
 			}
 		}
@@ -91,7 +97,7 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_li
 			lxm[2 * j + 1] = lx[2 * j + 1] - kZeroIdx2;
 		}
 		uint32_t lin = 0;
-		if constexpr(!TABLE) lin = __builtin_amdgcn_readfirstlane(s_lin[i % kPlainThreads]);
+		if constexpr(!TABLE) lin = __builtin_amdgcn_readfirstlane(s_lin[(i - i0) % kPlainThreads]);
 #pragma unroll
 		for(int r = 0; r < G; r++) {
 			uint32_t lc;
@@ -119,9 +125,15 @@ __device__ __forceinline__ void plain_tile(const uint16_t* s_exp, uint16_t* s_li
 	if(active) {
 #pragma unroll
 		for(int r = 0; r < G; r++) {
-			if(r0 + r < n_out)
-				*reinterpret_cast<uint4*>(out + (size_t)(r0 + r) * slot_words + w0) =
-				    make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+			if(r0 + r >= n_out) continue;
+			uint32_t* o = reinterpret_cast<uint32_t*>(out + (size_t)(r0 + r) * slot_words + w0);
+			if(partial) {
+#pragma unroll
+				for(int j = 0; j < 4; j++)
+					if(acc[r][j]) atomicXor(o + j, acc[r][j]);
+			} else {
+				*reinterpret_cast<uint4*>(o) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+			}
 		}
 	}
 }

@@ -251,8 +251,9 @@ hipError_t enqueue_compute(pph_serve* s, Job* j) {
 			if(e == hipSuccess) e = hipEventRecord(s->bl_tail, a);
 			if(e == hipSuccess) s->bl_any = true;
 		} else if(e == hipSuccess) {
+			// one workgroup per item: the engine's compute streams share the CUs
 			e = launch_muladd_plain((const uint16_t*)j->logd, j->slot / 2, (unsigned)j->n_in, j->logs, j->d_exps,
-			                        nullptr, 0, (uint16_t*)j->out, j->n_out, false, 0, *s->tab, a);
+			                        nullptr, 0, (uint16_t*)j->out, j->n_out, false, 0, *s->tab, a, false);
 		}
 		if(e == hipSuccess) e = launch_verify(j->out, j->slot, j->n_out, S, ck_offset(S), s->ck, j->valid, *s->tab, a);
 		// recovery packets hashed while the slices are still in HBM (lib/par2.js:505-513)
