@@ -62,10 +62,10 @@ FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 MFMA
 # DRAM bytes per mul-add launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by a 1 GiB plain copy in the same runs
 # (profiles/k_muladd_plain_2000x1MiB_R200.md), keyed by (form, inputs, slice, recovery).
-MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 17.18e9 + 0.210e9,
+MEASURED_TRAFFIC = {("plain", 2000, 1 << 20, 200): 27.29e9 + 0.655e9,
                     # k_blm_gemm<256>, the dominant kernel of the bilinear call
                     # (profiles/bilinear_32768x64KiB_R8192.md)
-                    ("bilinear", 32768, 1 << 16, 8192): 431.9e9 + 3.34e9}
+                    ("bilinear", 32768, 1 << 16, 8192): 435.6e9 + 2.77e9}
 
 CONFIGS = {
     # 0: the default run -- the plain workload's headline line, with the batched,
