@@ -181,6 +181,29 @@ def test_plain_random_vs_oracle(group):
         assert (valid == 1).all()
 
 
+def test_plain_no_inputs():
+    """A launch over no inputs: the sum of nothing, so overwriting zeroes the
+    recovery slots (the work split has no units to run) and accumulating
+    leaves them as they were."""
+    say("plain no inputs")
+    S = 20000
+    _, slot = P.slot_layout(S, 64)
+    n_out = 21
+    out = P.DeviceBuffer(slot * n_out)
+    before = np.full(slot * n_out, 0x5A, np.uint8)
+    out.upload(before)
+    d_exps = P.DeviceBuffer(2 * n_out)
+    d_exps.upload(np.arange(n_out, dtype=np.uint16))
+    parpar_amd.check(lib().pph_muladd_plain(out.ptr, slot, 0, None, d_exps.ptr, None, 0, out.ptr, n_out, 1, 0, None),
+                     "pph_muladd_plain accumulate")
+    parpar_amd.check(lib().pph_synchronize())
+    assert (out.download() == before).all()
+    parpar_amd.check(lib().pph_muladd_plain(out.ptr, slot, 0, None, d_exps.ptr, None, 0, out.ptr, n_out, 0, 0, None),
+                     "pph_muladd_plain overwrite")
+    parpar_amd.check(lib().pph_synchronize())
+    assert not out.download().any()
+
+
 def test_plain_explicit_coefficients():
     say("plain explicit")
     rng = np.random.default_rng(9)
