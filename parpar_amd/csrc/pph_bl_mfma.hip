@@ -220,8 +220,9 @@ __device__ __forceinline__ unsigned blm_divmod(unsigned n, unsigned d, unsigned 
 // Work item: component t, RT outputs r0 .. r0+RT-1 and 65536/RT words w0 ..
 // (256 x 256, waves 2x2; or 128 outputs x 512 words, waves 4x1 along the
 // words).  Each wave covers 128 words x 128 outputs = 4 x 4 blocks of 32 x 32.
-// The kernel is persistent: workgroup b runs items b, b + G, b + 2G, ... (G =
-// grid size) of one piece (`one`, table null) or of the pieces of `table`
+// The kernel is persistent: workgroup b runs items b and b + G (G = grid
+// size), then b + 2G, b + 3G, ... or, with `counter`, the items a counter
+// hands out, of one piece (`one`, table null) or of the pieces of `table`
 // (consecutive item ranges), one workgroup per CU, so the launch is paid once
 // per workgroup rather than once per item.
 // One step = 64 inputs (two 32-input groups, one MFMA k-step); every item has
