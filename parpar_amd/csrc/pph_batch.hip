@@ -469,9 +469,11 @@ extern "C" int pph_batch_create(const pph_job* jobs, unsigned n_jobs, unsigned c
 		b->predicted_s += pred[j];
 	}
 	// The bilinear jobs' workspace: all of their pieces at once where that fits
-	// a budget (48 GiB, at most the free memory less 8 GiB), else waves of
+	// a budget (96 GiB, at most the free memory less 8 GiB), else waves of
 	// them; jobs whose planes would take more than a quarter of it run alone.
-	size_t budget = (size_t)48 << 30;
+	// (The 64-job workload: 451-454 ms per replay with 96 or 160 GiB, 461 with
+	// 48 and 471 with 24: fewer waves.)
+	size_t budget = (size_t)96 << 30;
 	{
 		size_t fr = 0, tot = 0;
 		if(hipMemGetInfo(&fr, &tot) == hipSuccess && fr > ((size_t)8 << 30))
