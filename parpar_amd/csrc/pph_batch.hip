@@ -325,20 +325,21 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 		hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, a, b->d_counter, 0u);
 		e = hipGetLastError();
 	}
-	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
-		const pph_job& jb = b->jobs[j];
-		if(b->form[j] != PPH_FORM_PLAIN || !jb.n_in || !jb.n_out) continue;
-		e = launch_prep((uint8_t*)jb.in_slots, nullptr, jb.slot_bytes, jb.n_in, jb.in_lens, nullptr, jb.slice_size,
-		                ck_offset(jb.slice_size), b->ck, true, *b->tab, a);
-	}
+	// the dense jobs first: short launches that would hold CUs for tens of ms
+	// beside the bilinear GEMMs (a dense workgroup and a GEMM workgroup do not
+	// fit on one CU); then the plain jobs, whose launch runs as CUs free up
 	if(e == hipSuccess) e = launch_prep_data_jobs(b->ck_table(1), b->ck_n[1], b->ck_max[1], b->ck, 8, *b->tab, a);
-	// the dense launches (short) before the plain launch, which is the last
-	// on this stream and may still run when the first bilinear GEMM starts
 	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
 		const pph_job& jb = b->jobs[j];
 		if(b->form[j] != PPH_FORM_DENSE || !jb.n_out) continue;
 		e = launch_muladd_dense((const uint16_t*)jb.in_slots, jb.slot_bytes / 2, jb.n_in, jb.in_logs, jb.exps,
 		                        (uint16_t*)jb.out_slots, jb.n_out, false, *b->tab, a);
+	}
+	for(size_t j = 0; j < b->jobs.size() && e == hipSuccess; j++) {
+		const pph_job& jb = b->jobs[j];
+		if(b->form[j] != PPH_FORM_PLAIN || !jb.n_in || !jb.n_out) continue;
+		e = launch_prep((uint8_t*)jb.in_slots, nullptr, jb.slot_bytes, jb.n_in, jb.in_lens, nullptr, jb.slice_size,
+		                ck_offset(jb.slice_size), b->ck, true, *b->tab, a);
 	}
 	if(e == hipSuccess && b->n_items) {
 		dim3 grid(b->tab->cus > 0 ? b->tab->cus : 256), block(kPlainThreads);
@@ -352,12 +353,11 @@ static hipError_t enqueue_batch(pph_batch* b, hipStream_t s) {
 	}
 	// the join is recorded even after an error, so that a capture on `s` can end
 	const hipError_t ej = hipEventRecord(b->ev_join, a);
-	// The bilinear jobs' preparation takes the whole device (eight workgroups
-	// per CU), so that their GEMMs start as early as possible; the plain jobs'
-	// work gets CUs as they free up (measured on the 64-job workload: the
-	// first GEMM at 18 ms and 489 ms per replay, against 36 ms and 500 ms with
-	// two preparation workgroups per CU, which leave room for the plain
-	// launches beside them but move the bytes at half the rate).
+	// Bilinear jobs outside the plan (the plan prepares its own jobs wave by
+	// wave): their preparation on the whole device (eight workgroups per CU;
+	// two per CU leave room for the plain launches beside them but move the
+	// bytes at half the rate: 500 against 489 ms per replay on the 64-job
+	// workload, when all its bilinear jobs were prepared here).
 	hipError_t eb = launch_prep_data_jobs(b->ck_table(0), b->ck_n[0], b->ck_max[0], b->ck, 8, *b->tab, s);
 	if(eb == hipSuccess && b->timing) eb = hipEventRecord(b->ev_a, s);
 	if(eb == hipSuccess) eb = enqueue_bilinear(b, s);
