@@ -231,7 +231,10 @@ int pph_prep_slots_from(void* dst, const void* src, size_t slot_bytes, unsigned 
  * (coef_log2 == NULL; input_logs/exps are device uint16 arrays), or explicit:
  * coef_log2[i * coef_stride + r] = 2*log2(c), 0xFFFFFFFF for c == 0.
  * accumulate = 0 overwrites the outputs (first batch), 1 XORs into them.
- * group = outputs per pass (0 = auto). */
+ * group = outputs per pass (0 = auto).  The work is split evenly over the
+ * CUs; output tiles that two workgroups share are XORed together with
+ * device-scope atomics, so out_slots must be device memory, and launches on
+ * one output array must not overlap. */
 int pph_muladd_plain(const void* in_slots, size_t slot_bytes, unsigned n_in,
                      const uint16_t* input_logs, const uint16_t* exps,
                      const uint32_t* coef_log2, unsigned coef_stride,
